@@ -1,0 +1,107 @@
+"""ctypes binding of libfemhot.so (the C ABI declared in include/femhot.h).
+
+The library is built in-tree (folax_amd/libfemhot.so, see folax_amd/build.py).  There is no CPU
+fallback: if the shared library is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfemhot.so")
+
+QUAD4, HEX8, TET4 = 0, 1, 2
+THERMAL, MECHANICAL, NEOHOOKE = 0, 1, 2
+F32, F64 = 0, 1
+
+MASK_DIRICHLET = 0x1
+ELEMENT_FIELD = 0x2
+PICARD = 0x4
+TRANSPOSE = 0x8
+ACCUMULATE = 0x10
+SCATTER_COLOUR = 0x20
+SCATTER_CHUNK = 0x40
+
+OK, ERR_ARG, ERR_SHAPE, ERR_INDEX, ERR_NONFINITE, ERR_HIP, ERR_NOT_CONVERGED, ERR_UNSUPPORTED = range(8)
+
+ELEMENT_TYPES = {"quad": QUAD4, "hexahedron": HEX8, "tetra": TET4}
+
+# every symbol include/femhot.h declares
+EXPORTED = [
+    "femhot_last_error", "femhot_version", "femhot_ctx_create", "femhot_ctx_destroy", "femhot_ctx_info",
+    "femhot_pattern", "femhot_colouring", "femhot_residual", "femhot_tangent_csr", "femhot_residual_vjp",
+    "femhot_energy_grad", "femhot_pcg", "femhot_spmv", "femhot_orient_elements",
+]
+
+
+class FemhotError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"femhot error {code}: {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("elem_type", C.c_int), ("physics", C.c_int), ("num_gp", C.c_int),
+        ("young_modulus", C.c_double), ("poisson_ratio", C.c_double), ("beta", C.c_double), ("c", C.c_double),
+        ("body_force", C.c_double * 3),
+        ("n_nodes", C.c_longlong), ("n_elems", C.c_longlong),
+        ("coords", C.POINTER(C.c_double)), ("conn", C.POINTER(C.c_int)),
+        ("n_dirichlet", C.c_longlong), ("dirichlet_idx", C.POINTER(C.c_longlong)),
+        ("dirichlet_val", C.POINTER(C.c_double)),
+        ("n_segments", C.c_int), ("segment_offsets", C.POINTER(C.c_longlong)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load (once) and return the library handle; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import folax_amd.build as b; b.build()'`")
+    L = C.CDLL(LIB_PATH)
+    vp, ll, i32, u32, dbl = C.c_void_p, C.c_longlong, C.c_int, C.c_uint, C.c_double
+    L.femhot_last_error.restype = C.c_char_p
+    L.femhot_version.restype = C.c_char_p
+    L.femhot_ctx_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(vp)]
+    L.femhot_ctx_destroy.argtypes = [vp]
+    L.femhot_ctx_destroy.restype = None
+    L.femhot_ctx_info.argtypes = [vp, C.POINTER(ll), C.POINTER(i32), C.POINTER(i32), C.POINTER(ll), C.POINTER(i32),
+                                  C.POINTER(i32)]
+    L.femhot_pattern.argtypes = [vp, vp, vp, vp]
+    L.femhot_colouring.argtypes = [vp, C.POINTER(i32)]
+    L.femhot_residual.argtypes = [vp, i32, i32, vp, ll, vp, ll, vp, ll, vp, u32, vp]
+    L.femhot_tangent_csr.argtypes = [vp, i32, vp, vp, vp, vp, u32, vp]
+    L.femhot_residual_vjp.argtypes = [vp, i32, i32, vp, ll, vp, ll, vp, ll, vp, ll, vp, ll, u32, vp]
+    L.femhot_energy_grad.argtypes = [vp, i32, i32, vp, ll, vp, ll, vp, vp, ll, vp, ll, u32, vp]
+    L.femhot_pcg.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, i32, C.POINTER(i32), C.POINTER(dbl), vp]
+    L.femhot_spmv.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.femhot_orient_elements.argtypes = [i32, ll, C.POINTER(dbl), ll, C.POINTER(i32), C.POINTER(ll)]
+    for name in EXPORTED:
+        if name not in ("femhot_last_error", "femhot_version", "femhot_ctx_destroy"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise FemhotError(rc, lib().femhot_last_error().decode(errors="replace"))
+
+
+def orient_elements(elem_type, coords, conn):
+    """Mesh.CheckAndOrientElements (fol/mesh_input_output/mesh.py:123-142) through the C ABI (host
+    arrays, no GPU).  Returns (new_conn, n_swapped)."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    conn = np.array(conn, dtype=np.int32, order="C", copy=True)
+    n_sw = C.c_longlong(0)
+    check(lib().femhot_orient_elements(int(elem_type), coords.shape[0],
+                                       coords.ctypes.data_as(C.POINTER(C.c_double)), conn.shape[0],
+                                       conn.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n_sw)))
+    return conn, int(n_sw.value)

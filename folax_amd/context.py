@@ -1,0 +1,215 @@
+"""Thin, typed wrapper of one libfemhot context (one mesh + loss settings) over torch device memory.
+
+All buffers are torch tensors on the current CUDA (HIP) device; calls are enqueued on torch's
+current stream.  Inputs are validated on the host (device, dtype, shape, contiguity) before any
+launch; the library validates sizes again and reports errors through `FemhotError`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _dtype_code(t):
+    if t.dtype == torch.float32:
+        return L.F32
+    if t.dtype == torch.float64:
+        return L.F64
+    raise TypeError(f"femhot: unsupported dtype {t.dtype} (float32 or float64)")
+
+
+class FemhotContext:
+    """Owns a `femhot_ctx` built from host mesh arrays (fe_loss.py:57-122 Initialize)."""
+
+    def __init__(self, elem_type: str, physics: int, coords, conn, num_gp=0, young_modulus=1.0, poisson_ratio=0.3,
+                 beta=0.0, c=1.0, body_force=(0.0, 0.0, 0.0), dirichlet_idx=None, dirichlet_val=None,
+                 segment_offsets=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("femhot requires a ROCm GPU (torch.cuda.is_available() is False)")
+        lib = L.lib()
+        self._lib = lib
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        if coords.ndim != 2 or coords.shape[1] not in (2, 3):
+            raise ValueError("coords must be (n_nodes, 2|3)")
+        if coords.shape[1] == 2:
+            coords = np.concatenate([coords, np.zeros((coords.shape[0], 1))], axis=1)
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        self.elem_type = elem_type
+        self.et = L.ELEMENT_TYPES[elem_type]
+        self.physics = physics
+        self.n_nodes = coords.shape[0]
+        self.n_elems = conn.shape[0]
+        didx = np.ascontiguousarray(np.zeros(0) if dirichlet_idx is None else dirichlet_idx, dtype=np.int64)
+        dval = np.ascontiguousarray(np.zeros(0) if dirichlet_val is None else dirichlet_val, dtype=np.float64)
+        if didx.shape != dval.shape:
+            raise ValueError("dirichlet_idx and dirichlet_val must have the same length")
+        self._keep = (coords, conn, didx, dval)
+        d = L.MeshDesc()
+        d.elem_type, d.physics, d.num_gp = self.et, int(physics), int(num_gp)
+        d.young_modulus, d.poisson_ratio, d.beta, d.c = float(young_modulus), float(poisson_ratio), float(beta), float(c)
+        bf = list(np.asarray(body_force, dtype=np.float64).reshape(-1)) + [0.0, 0.0, 0.0]
+        d.body_force = (C.c_double * 3)(*bf[:3])
+        d.n_nodes, d.n_elems = self.n_nodes, self.n_elems
+        d.coords = coords.ctypes.data_as(C.POINTER(C.c_double))
+        d.conn = conn.ctypes.data_as(C.POINTER(C.c_int))
+        d.n_dirichlet = didx.shape[0]
+        d.dirichlet_idx = didx.ctypes.data_as(C.POINTER(C.c_longlong))
+        d.dirichlet_val = dval.ctypes.data_as(C.POINTER(C.c_double))
+        if segment_offsets is not None:
+            so = np.ascontiguousarray(segment_offsets, dtype=np.int64)
+            self._keep = self._keep + (so,)
+            d.n_segments = so.shape[0] - 1
+            d.segment_offsets = so.ctypes.data_as(C.POINTER(C.c_longlong))
+        else:
+            d.n_segments = 1
+        h = C.c_void_p()
+        L.check(lib.femhot_ctx_create(C.byref(d), C.byref(h)))
+        self._h = h
+        n_dofs, npe, dpn = C.c_longlong(), C.c_int(), C.c_int()
+        nseg = C.c_int()
+        L.check(lib.femhot_ctx_info(h, C.byref(n_dofs), C.byref(npe), C.byref(dpn), None, None, C.byref(nseg)))
+        self.n_dofs, self.nodes_per_elem, self.dofs_per_node = n_dofs.value, npe.value, dpn.value
+        self.n_segments = nseg.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.femhot_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------- structure ----------------
+    def nnz(self):
+        v = C.c_longlong()
+        L.check(self._lib.femhot_ctx_info(self._h, None, None, None, C.byref(v), None, None))
+        return v.value
+
+    def pattern(self, device=None):
+        """CSR pattern (row_ptr int64 (n_dofs+1), col_idx int32 (nnz)) on the device."""
+        device = device or torch.device("cuda")
+        nnz = self.nnz()
+        rp = torch.empty(self.n_dofs + 1, dtype=torch.int64, device=device)
+        ci = torch.empty(max(nnz, 1), dtype=torch.int32, device=device)
+        L.check(self._lib.femhot_pattern(self._h, _ptr(rp), _ptr(ci), _stream()))
+        return rp, ci[:nnz]
+
+    def colouring(self):
+        out = np.empty(self.n_elems, dtype=np.int32)
+        L.check(self._lib.femhot_colouring(self._h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    # ---------------- checks ----------------
+    def _check_in(self, t, name, last, batch, dtype, device):
+        if t is None:
+            raise ValueError(f"{name} is required")
+        if not t.is_cuda:
+            raise ValueError(f"{name} must be a device tensor")
+        if t.dtype != dtype:
+            raise TypeError(f"{name} has dtype {t.dtype}, expected {dtype}")
+        if t.device != device:
+            raise ValueError(f"{name} is on {t.device}, expected {device}")
+        if t.dim() != 2 or t.shape[0] != batch or t.shape[1] != last:
+            raise ValueError(f"{name} must have shape ({batch}, {last}), got {tuple(t.shape)}")
+        if t.stride(1) != 1:
+            raise ValueError(f"{name} must be contiguous in its last dimension")
+        return t.stride(0) if batch > 1 else last
+
+    def _field_size(self, flags):
+        return self.n_elems if flags & L.ELEMENT_FIELD else self.n_nodes
+
+    # ---------------- compute ----------------
+    def residual(self, field, u, want_R=True, want_energy=True, flags=0):
+        """R (B, n_dofs) and per-sample (per-segment) energies (float64)."""
+        u = u if u.dim() == 2 else u.unsqueeze(0)
+        field = field if field.dim() == 2 else field.unsqueeze(0)
+        B = u.shape[0]
+        dt, dev = u.dtype, u.device
+        us = self._check_in(u, "u", self.n_dofs, B, dt, dev)
+        fs = self._check_in(field, "field", self._field_size(flags), B, dt, dev)
+        R = torch.empty(B, self.n_dofs, dtype=dt, device=dev) if want_R else None
+        en = torch.empty(B * self.n_segments, dtype=torch.float64, device=dev) if want_energy else None
+        L.check(self._lib.femhot_residual(self._h, _dtype_code(u), B, _ptr(field), fs, _ptr(u), us, _ptr(R),
+                                          self.n_dofs, _ptr(en), flags, _stream()))
+        return R, (en.view(B, self.n_segments) if en is not None else None)
+
+    def residual_vjp(self, field, u, w, want_gu=True, want_gd=True, flags=0):
+        u = u if u.dim() == 2 else u.unsqueeze(0)
+        w = w if w.dim() == 2 else w.unsqueeze(0)
+        field = field if field.dim() == 2 else field.unsqueeze(0)
+        B = u.shape[0]
+        dt, dev = u.dtype, u.device
+        us = self._check_in(u, "u", self.n_dofs, B, dt, dev)
+        ws = self._check_in(w, "w", self.n_dofs, B, dt, dev)
+        fsz = self._field_size(flags)
+        fs = self._check_in(field, "field", fsz, B, dt, dev)
+        gu = torch.empty(B, self.n_dofs, dtype=dt, device=dev) if want_gu else None
+        gd = torch.empty(B, fsz, dtype=dt, device=dev) if want_gd else None
+        L.check(self._lib.femhot_residual_vjp(self._h, _dtype_code(u), B, _ptr(field), fs, _ptr(u), us, _ptr(w), ws,
+                                              _ptr(gu), self.n_dofs, _ptr(gd), fsz, flags, _stream()))
+        return gu, gd
+
+    def energy_grad(self, field, u, want_gu=True, want_gd=True, want_energy=True, flags=0, out=None):
+        u = u if u.dim() == 2 else u.unsqueeze(0)
+        field = field if field.dim() == 2 else field.unsqueeze(0)
+        B = u.shape[0]
+        dt, dev = u.dtype, u.device
+        us = self._check_in(u, "u", self.n_dofs, B, dt, dev)
+        fsz = self._field_size(flags)
+        fs = self._check_in(field, "field", fsz, B, dt, dev)
+        if out is not None:
+            en, gu, gd = out
+        else:
+            gu = torch.empty(B, self.n_dofs, dtype=dt, device=dev) if want_gu else None
+            gd = torch.empty(B, fsz, dtype=dt, device=dev) if want_gd else None
+            en = torch.empty(B * self.n_segments, dtype=torch.float64, device=dev) if want_energy else None
+        L.check(self._lib.femhot_energy_grad(self._h, _dtype_code(u), B, _ptr(field), fs, _ptr(u), us, _ptr(en),
+                                             _ptr(gu), self.n_dofs, _ptr(gd), fsz, flags, _stream()))
+        return (en.view(B, self.n_segments) if en is not None else None), gu, gd
+
+    def tangent_csr(self, field, u, want_R=True, flags=0):
+        """CSR values (nnz) on the pattern of `pattern()`, BC-masked, and the masked residual."""
+        field = field.reshape(-1)
+        u = u.reshape(-1)
+        dt, dev = u.dtype, u.device
+        self._check_in(u.view(1, -1), "u", self.n_dofs, 1, dt, dev)
+        self._check_in(field.view(1, -1), "field", self._field_size(flags), 1, dt, dev)
+        vals = torch.empty(max(self.nnz(), 1), dtype=dt, device=dev)
+        R = torch.empty(self.n_dofs, dtype=dt, device=dev) if want_R else None
+        L.check(self._lib.femhot_tangent_csr(self._h, _dtype_code(u), _ptr(field), _ptr(u), _ptr(vals), _ptr(R),
+                                             flags, _stream()))
+        return vals[: self.nnz()], R
+
+    def spmv(self, row_ptr, col_idx, vals, x):
+        y = torch.empty_like(x)
+        L.check(self._lib.femhot_spmv(self._h, _ptr(row_ptr), _ptr(col_idx), _ptr(vals), _ptr(x), _ptr(y), _stream()))
+        return y
+
+    def pcg(self, row_ptr, col_idx, vals, b, x0=None, tol=1e-10, maxiter=10000, mode="f64", raise_on_fail=True):
+        """Jacobi-PCG (FP64), or FP32 inner iterations + FP64 refinement (mode='mixed')."""
+        if vals.dtype != torch.float64 or b.dtype != torch.float64:
+            raise TypeError("pcg expects the FP64 tangent and right-hand side")
+        x = torch.zeros_like(b) if x0 is None else x0.clone()
+        it, rel = C.c_int(), C.c_double()
+        m = L.F64 if mode == "f64" else L.F32
+        rc = self._lib.femhot_pcg(self._h, m, _ptr(row_ptr), _ptr(col_idx), _ptr(vals), _ptr(b), _ptr(x), float(tol),
+                                  int(maxiter), C.byref(it), C.byref(rel), _stream())
+        if rc == L.ERR_NOT_CONVERGED and not raise_on_fail:
+            return x, it.value, rel.value, False
+        L.check(rc)
+        return x, it.value, rel.value, True

@@ -1,0 +1,279 @@
+// C-ABI compute entry points: residual/energy, residual VJP, potential gradient, tangent -> CSR.
+// Each validates its arguments on the host, clears its outputs with a fill kernel (graph-safe,
+// no memset nodes), picks the scatter strategy and launches on the caller's stream.
+#include <algorithm>
+
+#include "femhot_chunk.h"
+#include "femhot_launch.h"
+
+namespace femhot {
+
+static int launch_elem_any(int et, const ElemLaunch &L) {
+  switch (et) {
+    case QUAD4: return launch_elem<QUAD4>(L);
+    case HEX8: return launch_elem<HEX8>(L);
+    default: return launch_elem<TET4>(L);
+  }
+}
+
+static int launch_tangent_any(int et, const TangentLaunch &L) {
+  switch (et) {
+    case QUAD4: return launch_tangent<QUAD4>(L);
+    case HEX8: return launch_tangent<HEX8>(L);
+    default: return launch_tangent<TET4>(L);
+  }
+}
+
+struct CallIO {
+  int dtype = 0, batch = 1, mode = 0;
+  const void *field = nullptr;
+  long long fs = 0;
+  const void *u = nullptr;
+  long long us = 0;
+  const void *w = nullptr;
+  long long ws = 0;
+  void *ou = nullptr;
+  long long ous = 0;
+  void *od = nullptr;
+  long long ods = 0;
+  double *energy = nullptr;
+  unsigned flags = 0;
+  hipStream_t stream = nullptr;
+};
+
+static int validate(Ctx *c, const CallIO &io, const char *fn) {
+  const bool elemental = (io.flags & FEMHOT_ELEMENT_FIELD) != 0;
+  const long long fsize = elemental ? c->Ne : c->Nn;
+  std::string f(fn);
+  FH_CHECK_ARG(io.dtype == FEMHOT_F32 || io.dtype == FEMHOT_F64, FEMHOT_ERR_ARG, f + ": dtype must be F32 or F64");
+  FH_CHECK_ARG(io.batch >= 1 && io.batch <= 65535, FEMHOT_ERR_SHAPE, f + ": batch must be in [1, 65535]");
+  FH_CHECK_ARG(io.field && io.u, FEMHOT_ERR_ARG, f + ": null field/u");
+  if (io.batch > 1) {
+    FH_CHECK_ARG(io.fs >= fsize && io.us >= c->Ndof, FEMHOT_ERR_SHAPE, f + ": input strides smaller than rows");
+    FH_CHECK_ARG(!io.ou || io.ous >= c->Ndof, FEMHOT_ERR_SHAPE, f + ": output stride smaller than n_dofs");
+    FH_CHECK_ARG(!io.od || io.ods >= fsize, FEMHOT_ERR_SHAPE, f + ": g_field stride smaller than field size");
+    FH_CHECK_ARG(!io.w || io.ws >= c->Ndof, FEMHOT_ERR_SHAPE, f + ": cotangent stride smaller than n_dofs");
+  }
+  return FEMHOT_OK;
+}
+
+static int run_elem(Ctx *c, const CallIO &io) {
+  const bool elemental = (io.flags & FEMHOT_ELEMENT_FIELD) != 0;
+  const long long fsize = elemental ? c->Ne : c->Nn;
+  const bool acc = (io.flags & FEMHOT_ACCUMULATE) != 0;
+  int rc;
+  // clear outputs
+  if (!acc) {
+    if (io.dtype == FEMHOT_F32) {
+      if ((rc = fill<float>((float *)io.ou, c->Ndof, io.batch > 1 ? io.ous : c->Ndof, io.batch, 0.f, io.stream)))
+        return rc;
+      if (!elemental || io.mode == MODE_RES)
+        if ((rc = fill<float>((float *)io.od, fsize, io.batch > 1 ? io.ods : fsize, io.batch, 0.f, io.stream)))
+          return rc;
+    } else {
+      if ((rc = fill<double>((double *)io.ou, c->Ndof, io.batch > 1 ? io.ous : c->Ndof, io.batch, 0.0, io.stream)))
+        return rc;
+      if (!elemental || io.mode == MODE_RES)
+        if ((rc = fill<double>((double *)io.od, fsize, io.batch > 1 ? io.ods : fsize, io.batch, 0.0, io.stream)))
+          return rc;
+    }
+    if ((rc = fill<double>(io.energy, (long long)io.batch * c->nseg, (long long)io.batch * c->nseg, 1, 0.0,
+                           io.stream)))
+      return rc;
+  }
+  if (io.flags & FEMHOT_SCATTER_CHUNK) {
+    return run_chunk(c, io.dtype, io.mode, io.batch, io.field, io.fs, io.u, io.us, io.w, io.ws, io.ou, io.ous, io.od,
+                     io.ods, io.energy, io.flags, io.stream, 0, 0);
+  }
+  ElemLaunch L{};
+  L.dtype = io.dtype;
+  L.ph = c->ph;
+  L.ngp = c->ngp;
+  L.mode = io.mode;
+  L.stream = io.stream;
+  KArgs &A = L.args;
+  A.coords = c->d_coords64;
+  A.conn = c->d_conn;
+  A.Ne = c->Ne;
+  A.Nn = c->Nn;
+  A.field = io.field;
+  A.fstride = io.batch > 1 ? io.fs : 0;
+  A.u = io.u;
+  A.ustride = io.batch > 1 ? io.us : 0;
+  A.w = io.w;
+  A.wstride = io.batch > 1 ? io.ws : 0;
+  A.out_u = io.ou;
+  A.oustride = io.batch > 1 ? io.ous : 0;
+  A.out_d = io.od;
+  A.odstride = io.batch > 1 ? io.ods : 0;
+  A.energy = io.energy;
+  A.dmask = c->d_dmask;
+  A.seg_off = c->d_seg_off;
+  A.nseg = c->nseg;
+  A.flags = io.flags;
+  A.pp = c->pp;
+  if (io.flags & FEMHOT_SCATTER_COLOUR) {
+    if ((rc = ensure_colouring(c))) return rc;
+    L.plain = true;
+    for (int k = 0; k < c->n_colours; ++k) {
+      const long long cnt = c->h_colour_off[k + 1] - c->h_colour_off[k];
+      if (!cnt) continue;
+      A.list = c->d_colour_elems + c->h_colour_off[k];
+      A.count = cnt;
+      L.grid = dim3((unsigned)((cnt + 255) / 256), io.batch);
+      if ((rc = launch_elem_any(c->et, L))) return rc;
+    }
+    return FEMHOT_OK;
+  }
+  L.plain = false;
+  A.list = nullptr;
+  A.count = c->Ne;
+  L.grid = dim3((unsigned)((c->Ne + 255) / 256), io.batch);
+  return launch_elem_any(c->et, L);
+}
+
+}  // namespace femhot
+
+using namespace femhot;
+
+extern "C" {
+
+int femhot_residual(femhot_ctx *h, int dtype, int batch, const void *field, long long fs, const void *u,
+                    long long us, void *R, long long Rs, double *energy, unsigned flags, void *stream) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  CallIO io;
+  io.dtype = dtype;
+  io.batch = batch;
+  io.mode = MODE_RES;
+  io.field = field;
+  io.fs = fs;
+  io.u = u;
+  io.us = us;
+  io.ou = R;
+  io.ous = Rs;
+  io.energy = energy;
+  io.flags = flags;
+  io.stream = (hipStream_t)stream;
+  int rc = validate(&h->c, io, "femhot_residual");
+  if (rc) return rc;
+  return run_elem(&h->c, io);
+}
+
+int femhot_residual_vjp(femhot_ctx *h, int dtype, int batch, const void *field, long long fs, const void *u,
+                        long long us, const void *w, long long ws, void *g_u, long long gus, void *g_field,
+                        long long gfs, unsigned flags, void *stream) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  FH_CHECK_ARG(w, FEMHOT_ERR_ARG, "femhot_residual_vjp: null cotangent");
+  CallIO io;
+  io.dtype = dtype;
+  io.batch = batch;
+  io.mode = MODE_VJP;
+  io.field = field;
+  io.fs = fs;
+  io.u = u;
+  io.us = us;
+  io.w = w;
+  io.ws = ws;
+  io.ou = g_u;
+  io.ous = gus;
+  io.od = g_field;
+  io.ods = gfs;
+  io.flags = flags;
+  io.stream = (hipStream_t)stream;
+  int rc = validate(&h->c, io, "femhot_residual_vjp");
+  if (rc) return rc;
+  return run_elem(&h->c, io);
+}
+
+int femhot_energy_grad(femhot_ctx *h, int dtype, int batch, const void *field, long long fs, const void *u,
+                       long long us, double *energy, void *g_u, long long gus, void *g_field, long long gfs,
+                       unsigned flags, void *stream) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  Ctx *c = &h->c;
+  FH_CHECK_ARG(!(c->ph == THERMAL && c->pp.beta != 0.0), FEMHOT_ERR_UNSUPPORTED,
+               "femhot_energy_grad: thermal conduction with beta != 0 has no potential");
+  CallIO io;
+  io.dtype = dtype;
+  io.batch = batch;
+  io.mode = MODE_EGRAD;
+  io.field = field;
+  io.fs = fs;
+  io.u = u;
+  io.us = us;
+  io.ou = g_u;
+  io.ous = gus;
+  io.od = g_field;
+  io.ods = gfs;
+  io.energy = energy;
+  io.flags = flags;
+  io.stream = (hipStream_t)stream;
+  int rc = validate(c, io, "femhot_energy_grad");
+  if (rc) return rc;
+  return run_elem(c, io);
+}
+
+int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *u, void *vals, void *R,
+                       unsigned flags, void *stream) {
+  FH_CHECK_ARG(h && field && u && vals, FEMHOT_ERR_ARG, "femhot_tangent_csr: null argument");
+  FH_CHECK_ARG(dtype == FEMHOT_F32 || dtype == FEMHOT_F64, FEMHOT_ERR_ARG, "femhot_tangent_csr: bad dtype");
+  Ctx *c = &h->c;
+  int rc = ensure_pattern(c);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const bool acc = (flags & FEMHOT_ACCUMULATE) != 0;
+  if (!acc) {
+    rc = dtype == FEMHOT_F32 ? fill<float>((float *)vals, c->nnz, c->nnz, 1, 0.f, s)
+                             : fill<double>((double *)vals, c->nnz, c->nnz, 1, 0.0, s);
+    if (rc) return rc;
+  }
+  TangentLaunch L{};
+  L.dtype = dtype;
+  L.ph = c->ph;
+  L.ngp = c->ngp;
+  L.stream = s;
+  TArgs &A = L.args;
+  A.coords = c->d_coords64;
+  A.conn = c->d_conn;
+  A.field = field;
+  A.u = u;
+  A.vals = vals;
+  A.row_ptr = c->d_row_ptr;
+  A.slot = c->d_slot;
+  A.dmask = c->d_dmask;
+  A.flags = flags;
+  A.pp = c->pp;
+  const int n = c->n;
+  if (flags & FEMHOT_SCATTER_COLOUR) {
+    if ((rc = ensure_colouring(c))) return rc;
+    L.plain = true;
+    for (int k = 0; k < c->n_colours; ++k) {
+      const long long cnt = c->h_colour_off[k + 1] - c->h_colour_off[k];
+      if (!cnt) continue;
+      A.list = c->d_colour_elems + c->h_colour_off[k];
+      A.count = cnt;
+      L.grid = dim3((unsigned)((cnt * n + 255) / 256));
+      if ((rc = launch_tangent_any(c->et, L))) return rc;
+    }
+  } else {
+    L.plain = false;
+    A.list = nullptr;
+    A.count = c->Ne;
+    L.grid = dim3((unsigned)((c->Ne * n + 255) / 256));
+    if ((rc = launch_tangent_any(c->et, L))) return rc;
+  }
+  if (R) {
+    CallIO io;
+    io.dtype = dtype;
+    io.batch = 1;
+    io.mode = MODE_RES;
+    io.field = field;
+    io.u = u;
+    io.ou = R;
+    io.flags = (flags & (FEMHOT_ELEMENT_FIELD | FEMHOT_ACCUMULATE | FEMHOT_SCATTER_COLOUR)) | FEMHOT_MASK_DIRICHLET;
+    io.stream = s;
+    if ((rc = run_elem(c, io))) return rc;
+  }
+  return FEMHOT_OK;
+}
+
+}  // extern "C"

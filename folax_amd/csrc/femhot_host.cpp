@@ -1,0 +1,512 @@
+// Host side of the femhot library: context creation, mesh validation, CSR pattern, slot table,
+// colouring, chunk plans and error reporting.  Built with hipcc as host code into libfemhot.so.
+//
+// Reference behaviour followed:
+//   fe_loss.py:34-55    Dirichlet dof numbering ndpn*node + comp
+//   fe_loss.py:57-122   Initialize: element type, Gauss rule defaults (num_gp), dofs per node
+//   fe_loss.py:178-184, 313-316 + fe_solver.py:71-72
+//                       COO of element blocks -> scipy csr_array (summed duplicates, sorted
+//                       columns, explicit zeros kept): the pattern built here
+//   mesh.py:123-142     orientation fix (swap local nodes 0 and 1 where detJ < 0)
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "femhot_internal.h"
+
+namespace femhot {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+const std::string &get_error() { return g_err; }
+
+static int default_num_gp(int et) { return et == TET4 ? 1 : 2; }
+
+// Build node adjacency (sorted unique neighbour lists incl. self) and the CSR pattern + slot table.
+int ensure_pattern(Ctx *c) {
+  if (c->have_pattern) return FEMHOT_OK;
+  const int n = c->n, P = c->ndpn;
+  const long long Nn = c->Nn, Ne = c->Ne;
+  std::vector<long long> cnt(Nn + 1, 0);
+  for (long long e = 0; e < Ne; ++e)
+    for (int a = 0; a < n; ++a) cnt[c->h_conn[e * n + a] + 1]++;
+  for (long long i = 0; i < Nn; ++i) cnt[i + 1] += cnt[i];
+  std::vector<long long> n2e(cnt[Nn]);
+  {
+    std::vector<long long> pos(cnt.begin(), cnt.end() - 1);
+    for (long long e = 0; e < Ne; ++e)
+      for (int a = 0; a < n; ++a) n2e[pos[c->h_conn[e * n + a]]++] = e;
+  }
+  std::vector<long long> adj_off(Nn + 1, 0);
+  std::vector<int> adj;
+  adj.reserve((size_t)Nn * 16);
+  std::vector<int> tmp;
+  for (long long i = 0; i < Nn; ++i) {
+    tmp.clear();
+    for (long long k = cnt[i]; k < cnt[i + 1]; ++k) {
+      const long long e = n2e[k];
+      for (int b = 0; b < n; ++b) tmp.push_back(c->h_conn[e * n + b]);
+    }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    adj.insert(adj.end(), tmp.begin(), tmp.end());
+    adj_off[i + 1] = (long long)adj.size();
+  }
+  // CSR rows: node-major, P rows per node, each with P*deg columns (neighbour m, comp d) sorted
+  c->h_row_ptr.assign(c->Ndof + 1, 0);
+  long long base = 0;
+  for (long long i = 0; i < Nn; ++i) {
+    const long long deg = adj_off[i + 1] - adj_off[i];
+    for (int cc = 0; cc < P; ++cc) c->h_row_ptr[i * P + cc] = base + (long long)cc * P * deg;
+    base += (long long)P * P * deg;
+  }
+  c->h_row_ptr[c->Ndof] = base;
+  c->nnz = base;
+  if (base > 2147483647LL * 8) {
+    set_error("ensure_pattern: tangent too large");
+    return FEMHOT_ERR_SHAPE;
+  }
+  c->h_col_idx.resize(base);
+  for (long long i = 0; i < Nn; ++i) {
+    const long long deg = adj_off[i + 1] - adj_off[i];
+    for (int cc = 0; cc < P; ++cc) {
+      long long r = c->h_row_ptr[i * P + cc];
+      for (long long k = 0; k < deg; ++k)
+        for (int d = 0; d < P; ++d) c->h_col_idx[r++] = adj[adj_off[i] + k] * P + d;
+    }
+  }
+  // slot table: position of node b in the sorted neighbour list of node a, per element
+  std::vector<uint8_t> slot((size_t)Ne * n * n);
+  for (long long e = 0; e < Ne; ++e)
+    for (int a = 0; a < n; ++a) {
+      const int na = c->h_conn[e * n + a];
+      const int *beg = adj.data() + adj_off[na], *end = adj.data() + adj_off[na + 1];
+      for (int b = 0; b < n; ++b) {
+        const int nb = c->h_conn[e * n + b];
+        const long long k = std::lower_bound(beg, end, nb) - beg;
+        if (k > 255) {
+          set_error("ensure_pattern: node with more than 256 neighbours is not supported");
+          return FEMHOT_ERR_UNSUPPORTED;
+        }
+        slot[((size_t)e * n + a) * n + b] = (uint8_t)k;
+      }
+    }
+  FH_HIP(hipMalloc(&c->d_row_ptr, sizeof(long long) * (c->Ndof + 1)));
+  FH_HIP(hipMalloc(&c->d_col_idx, sizeof(int) * std::max<long long>(1, base)));
+  FH_HIP(hipMalloc(&c->d_slot, slot.size()));
+  FH_HIP(hipMemcpy(c->d_row_ptr, c->h_row_ptr.data(), sizeof(long long) * (c->Ndof + 1), hipMemcpyHostToDevice));
+  if (base) FH_HIP(hipMemcpy(c->d_col_idx, c->h_col_idx.data(), sizeof(int) * base, hipMemcpyHostToDevice));
+  FH_HIP(hipMemcpy(c->d_slot, slot.data(), slot.size(), hipMemcpyHostToDevice));
+  c->have_pattern = true;
+  return FEMHOT_OK;
+}
+
+// Deterministic greedy first-fit colouring in element order over the shared-node graph.
+int ensure_colouring(Ctx *c) {
+  if (c->have_colour) return FEMHOT_OK;
+  const int n = c->n;
+  constexpr int W = 4;  // up to 256 colours
+  std::vector<uint64_t> used((size_t)c->Nn * W, 0);
+  c->h_colour.assign(c->Ne, 0);
+  int ncol = 0;
+  for (long long e = 0; e < c->Ne; ++e) {
+    uint64_t m[W] = {0, 0, 0, 0};
+    for (int a = 0; a < n; ++a) {
+      const uint64_t *u = &used[(size_t)c->h_conn[e * n + a] * W];
+      for (int w = 0; w < W; ++w) m[w] |= u[w];
+    }
+    int col = -1;
+    for (int w = 0; w < W && col < 0; ++w)
+      if (~m[w]) col = w * 64 + __builtin_ctzll(~m[w]);
+    if (col < 0) {
+      set_error("ensure_colouring: more than 256 colours needed");
+      return FEMHOT_ERR_UNSUPPORTED;
+    }
+    c->h_colour[e] = col;
+    ncol = std::max(ncol, col + 1);
+    for (int a = 0; a < n; ++a) used[(size_t)c->h_conn[e * n + a] * W + col / 64] |= (1ULL << (col % 64));
+  }
+  c->n_colours = ncol;
+  c->h_colour_off.assign(ncol + 1, 0);
+  for (long long e = 0; e < c->Ne; ++e) c->h_colour_off[c->h_colour[e] + 1]++;
+  for (int k = 0; k < ncol; ++k) c->h_colour_off[k + 1] += c->h_colour_off[k];
+  std::vector<int> order(c->Ne);
+  {
+    std::vector<int> pos(c->h_colour_off.begin(), c->h_colour_off.end() - 1);
+    for (long long e = 0; e < c->Ne; ++e) order[pos[c->h_colour[e]]++] = (int)e;
+  }
+  FH_HIP(hipMalloc(&c->d_colour_elems, sizeof(int) * std::max<long long>(1, c->Ne)));
+  FH_HIP(hipMemcpy(c->d_colour_elems, order.data(), sizeof(int) * c->Ne, hipMemcpyHostToDevice));
+  c->have_colour = true;
+  return FEMHOT_OK;
+}
+
+static inline uint64_t spread3(uint64_t x) {  // 21 bits -> every third bit
+  x &= 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffULL;
+  x = (x | x << 16) & 0x1f0000ff0000ffULL;
+  x = (x | x << 8) & 0x100f00f00f00f00fULL;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+  x = (x | x << 2) & 0x1249249249249249ULL;
+  return x;
+}
+
+static void free_chunks(ChunkPlan &p) {
+  (void)hipFree(p.chunk_elem_off);
+  (void)hipFree(p.chunk_node_off);
+  (void)hipFree(p.chunk_seg);
+  (void)hipFree(p.perm);
+  (void)hipFree(p.lconn);
+  (void)hipFree(p.chunk_nodes);
+  (void)hipFree(p.shared_nodes);
+  p = ChunkPlan{};
+}
+
+// Chunk plan: elements ordered by (segment, Morton code of the centroid) and cut into runs of at
+// most `epc` elements; each chunk lists its nodes (sorted), its elements' chunk-local connectivity
+// and which of its nodes are shared with other chunks.
+int ensure_chunks(Ctx *c, int epc) {
+  if (c->chunk.n_chunks && c->chunk.epc == epc) return FEMHOT_OK;
+  if (epc < 1 || epc > 8192) {
+    set_error("ensure_chunks: elements per chunk must be in [1, 8192]");
+    return FEMHOT_ERR_ARG;
+  }
+  free_chunks(c->chunk);
+  const int n = c->n, D = c->dim;
+  const long long Ne = c->Ne;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (long long i = 0; i < c->Nn; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], c->h_coords[i * 3 + k]);
+      hi[k] = std::max(hi[k], c->h_coords[i * 3 + k]);
+    }
+  std::vector<int> seg_of(Ne, 0);
+  for (int s = 0; s < c->nseg; ++s)
+    for (long long e = c->h_seg_off[s]; e < c->h_seg_off[s + 1]; ++e) seg_of[e] = s;
+  std::vector<std::pair<uint64_t, int>> key(Ne);
+  for (long long e = 0; e < Ne; ++e) {
+    uint64_t q[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) {
+      double s = 0;
+      for (int a = 0; a < n; ++a) s += c->h_coords[(size_t)c->h_conn[e * n + a] * 3 + k];
+      s /= n;
+      const double span = hi[k] - lo[k];
+      const double t = span > 0 ? (s - lo[k]) / span : 0.0;
+      q[k] = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097151.0));
+    }
+    const uint64_t m = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    key[e] = {m, (int)e};
+  }
+  std::stable_sort(key.begin(), key.end(), [&](const auto &x, const auto &y) {
+    const int sx = seg_of[x.second], sy = seg_of[y.second];
+    return sx != sy ? sx < sy : x.first < y.first;
+  });
+  std::vector<int> perm(Ne), elem_off{0}, node_off{0}, seg, nodes;
+  std::vector<uint16_t> lconn((size_t)Ne * n);
+  for (long long i = 0; i < Ne; ++i) perm[i] = key[i].second;
+  long long i = 0;
+  std::vector<int> loc;
+  int max_nodes = 0, max_elems = 0;
+  while (i < Ne) {
+    long long j = i;
+    const int s = seg_of[perm[i]];
+    while (j < Ne && j - i < epc && seg_of[perm[j]] == s) ++j;
+    loc.clear();
+    for (long long k = i; k < j; ++k)
+      for (int a = 0; a < n; ++a) loc.push_back(c->h_conn[(size_t)perm[k] * n + a]);
+    std::sort(loc.begin(), loc.end());
+    loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+    for (long long k = i; k < j; ++k)
+      for (int a = 0; a < n; ++a) {
+        const int g = c->h_conn[(size_t)perm[k] * n + a];
+        lconn[(size_t)k * n + a] = (uint16_t)(std::lower_bound(loc.begin(), loc.end(), g) - loc.begin());
+      }
+    nodes.insert(nodes.end(), loc.begin(), loc.end());
+    max_nodes = std::max<int>(max_nodes, (int)loc.size());
+    max_elems = std::max<int>(max_elems, (int)(j - i));
+    elem_off.push_back((int)j);
+    node_off.push_back((int)nodes.size());
+    seg.push_back(s);
+    i = j;
+  }
+  std::vector<int> touch(c->Nn, 0);
+  for (int g : nodes) touch[g]++;
+  std::vector<int> shared;
+  for (long long g = 0; g < c->Nn; ++g)
+    if (touch[g] > 1) shared.push_back((int)g);
+  for (int &g : nodes)
+    if (touch[g] > 1) g = (int)((unsigned)g | 0x80000000u);
+  ChunkPlan &p = c->chunk;
+  p.epc = epc;
+  p.n_chunks = (int)seg.size();
+  p.max_nodes = max_nodes;
+  p.max_elems = max_elems;
+  p.n_shared = (long long)shared.size();
+  FH_HIP(hipMalloc(&p.chunk_elem_off, sizeof(int) * elem_off.size()));
+  FH_HIP(hipMalloc(&p.chunk_node_off, sizeof(int) * node_off.size()));
+  FH_HIP(hipMalloc(&p.chunk_seg, sizeof(int) * std::max<size_t>(1, seg.size())));
+  FH_HIP(hipMalloc(&p.perm, sizeof(int) * std::max<long long>(1, Ne)));
+  FH_HIP(hipMalloc(&p.lconn, sizeof(uint16_t) * std::max<size_t>(1, lconn.size())));
+  FH_HIP(hipMalloc(&p.chunk_nodes, sizeof(int) * std::max<size_t>(1, nodes.size())));
+  FH_HIP(hipMalloc(&p.shared_nodes, sizeof(int) * std::max<size_t>(1, shared.size())));
+  FH_HIP(hipMemcpy(p.chunk_elem_off, elem_off.data(), sizeof(int) * elem_off.size(), hipMemcpyHostToDevice));
+  FH_HIP(hipMemcpy(p.chunk_node_off, node_off.data(), sizeof(int) * node_off.size(), hipMemcpyHostToDevice));
+  if (!seg.empty()) FH_HIP(hipMemcpy(p.chunk_seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
+  if (Ne) FH_HIP(hipMemcpy(p.perm, perm.data(), sizeof(int) * Ne, hipMemcpyHostToDevice));
+  if (!lconn.empty())
+    FH_HIP(hipMemcpy(p.lconn, lconn.data(), sizeof(uint16_t) * lconn.size(), hipMemcpyHostToDevice));
+  if (!nodes.empty())
+    FH_HIP(hipMemcpy(p.chunk_nodes, nodes.data(), sizeof(int) * nodes.size(), hipMemcpyHostToDevice));
+  if (!shared.empty())
+    FH_HIP(hipMemcpy(p.shared_nodes, shared.data(), sizeof(int) * shared.size(), hipMemcpyHostToDevice));
+  return FEMHOT_OK;
+}
+
+int ensure_workspace(Ctx *c, size_t bytes) {
+  if (c->ws_bytes >= bytes) return FEMHOT_OK;
+  if (c->d_ws) FH_HIP(hipFree(c->d_ws));
+  c->d_ws = nullptr;
+  c->ws_bytes = 0;
+  FH_HIP(hipMalloc(&c->d_ws, bytes));
+  c->ws_bytes = bytes;
+  return FEMHOT_OK;
+}
+
+static double det_at_center(int et, const double *X) {
+  // J at the one-point rule (mesh.py:128-132 uses GetIntegrationData of the default rule)
+  const int n = nodes_per_elem(et), D = elem_dim(et);
+  double J[3][3] = {{0}};
+  double gx, gy, gz, w;
+  if (et == TET4)
+    gauss_point<TET4, 1>(0, gx, gy, gz, w);
+  else
+    gx = gy = gz = 0.0;
+  for (int a = 0; a < n; ++a) {
+    double d[3];
+    if (et == QUAD4) {
+      const int sx[4] = {-1, 1, 1, -1}, sy[4] = {-1, -1, 1, 1};
+      d[0] = 0.25 * sx[a] * (1 + sy[a] * gy);
+      d[1] = 0.25 * sy[a] * (1 + sx[a] * gx);
+    } else if (et == HEX8) {
+      const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+      d[0] = 0.125 * sx[a] * (1 + sy[a] * gy) * (1 + sz[a] * gz);
+      d[1] = 0.125 * sy[a] * (1 + sx[a] * gx) * (1 + sz[a] * gz);
+      d[2] = 0.125 * sz[a] * (1 + sx[a] * gx) * (1 + sy[a] * gy);
+    } else {
+      const double t[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+      d[0] = t[a][0];
+      d[1] = t[a][1];
+      d[2] = t[a][2];
+    }
+    for (int i = 0; i < D; ++i)
+      for (int k = 0; k < D; ++k) J[i][k] += X[a * 3 + i] * d[k];
+  }
+  if (D == 2) return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+}
+
+}  // namespace femhot
+
+using namespace femhot;
+
+extern "C" {
+
+const char *femhot_last_error(void) { return get_error().c_str(); }
+const char *femhot_version(void) { return "femhot 0.1 gfx950"; }
+
+int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
+  FH_CHECK_ARG(d && out, FEMHOT_ERR_ARG, "null argument");
+  *out = nullptr;
+  FH_CHECK_ARG(d->elem_type >= QUAD4 && d->elem_type <= TET4, FEMHOT_ERR_ARG, "unknown element type");
+  FH_CHECK_ARG(d->physics >= THERMAL && d->physics <= NEOHOOKE, FEMHOT_ERR_ARG, "unknown physics");
+  FH_CHECK_ARG(d->num_gp >= 0 && d->num_gp <= 3, FEMHOT_ERR_ARG, "num_gp must be 0 (default), 1, 2 or 3");
+  FH_CHECK_ARG(d->n_nodes > 0 && d->n_elems > 0, FEMHOT_ERR_SHAPE, "empty mesh");
+  FH_CHECK_ARG(d->n_nodes < (1LL << 31) && d->n_elems < (1LL << 31), FEMHOT_ERR_SHAPE, "mesh too large for int32");
+  FH_CHECK_ARG(d->coords && d->conn, FEMHOT_ERR_ARG, "null coords/conn");
+  FH_CHECK_ARG(d->n_dirichlet == 0 || (d->dirichlet_idx && d->dirichlet_val), FEMHOT_ERR_ARG,
+               "null Dirichlet arrays");
+  auto *h = new femhot_ctx();
+  Ctx &c = h->c;
+  c.desc = *d;
+  c.et = d->elem_type;
+  c.ph = d->physics;
+  c.ngp = d->num_gp ? d->num_gp : default_num_gp(c.et);
+  c.n = nodes_per_elem(c.et);
+  c.dim = elem_dim(c.et);
+  c.ndpn = c.ph == THERMAL ? 1 : c.dim;
+  c.Nn = d->n_nodes;
+  c.Ne = d->n_elems;
+  c.Ndof = c.Nn * c.ndpn;
+  c.h_coords.assign(d->coords, d->coords + c.Nn * 3);
+  c.h_conn.assign(d->conn, d->conn + c.Ne * c.n);
+  for (long long i = 0; i < c.Ne * c.n; ++i)
+    if (c.h_conn[i] < 0 || c.h_conn[i] >= c.Nn) {
+      delete h;
+      set_error("femhot_ctx_create: connectivity index out of range");
+      return FEMHOT_ERR_INDEX;
+    }
+  for (long long i = 0; i < c.Nn * 3; ++i)
+    if (!std::isfinite(c.h_coords[i])) {
+      delete h;
+      set_error("femhot_ctx_create: non-finite coordinate");
+      return FEMHOT_ERR_NONFINITE;
+    }
+  c.h_dir_idx.assign(d->dirichlet_idx, d->dirichlet_idx + d->n_dirichlet);
+  c.h_dir_val.assign(d->dirichlet_val, d->dirichlet_val + d->n_dirichlet);
+  for (long long v : c.h_dir_idx)
+    if (v < 0 || v >= c.Ndof) {
+      delete h;
+      set_error("femhot_ctx_create: Dirichlet dof index out of range");
+      return FEMHOT_ERR_INDEX;
+    }
+  c.nseg = d->n_segments > 1 ? d->n_segments : 1;
+  if (c.nseg > 1) {
+    if (!d->segment_offsets) {
+      delete h;
+      set_error("femhot_ctx_create: segment_offsets required when n_segments > 1");
+      return FEMHOT_ERR_ARG;
+    }
+    c.h_seg_off.assign(d->segment_offsets, d->segment_offsets + c.nseg + 1);
+    bool ok = c.h_seg_off[0] == 0 && c.h_seg_off[c.nseg] == c.Ne;
+    for (int s = 0; s < c.nseg; ++s) ok = ok && c.h_seg_off[s] <= c.h_seg_off[s + 1];
+    if (!ok) {
+      delete h;
+      set_error("femhot_ctx_create: segment_offsets must be non-decreasing from 0 to n_elems");
+      return FEMHOT_ERR_SHAPE;
+    }
+  } else {
+    c.h_seg_off = {0, c.Ne};
+  }
+  // physics parameters (mechanical.py:138-160, mechanical_neohooke.py:253-255)
+  PhysParams &p = c.pp;
+  p.E = d->young_modulus;
+  p.nu = d->poisson_ratio;
+  p.beta = d->beta;
+  p.c = d->c;
+  for (int k = 0; k < 3; ++k) p.body[k] = d->body_force[k];
+  const double E = p.E, nu = p.nu;
+  const double c1 = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  p.c2 = c1 * (1.0 - nu);
+  p.c3 = c1 * nu;
+  p.c4 = c1 * 0.5 * (1.0 - 2.0 * nu);
+  p.f2 = E / (1.0 - nu * nu);
+  p.f2nu = p.f2 * nu;
+  p.f2sh = p.f2 * (1.0 - nu) / 2.0;
+  p.kap1 = 1.0 / (3.0 * (1.0 - 2.0 * nu));
+  p.mu1 = 1.0 / (2.0 * (1.0 + nu));
+  // device copies
+  std::vector<float> c32(c.h_coords.begin(), c.h_coords.end());
+  std::vector<uint8_t> dmask(c.Ndof, 0);
+  for (long long v : c.h_dir_idx) dmask[v] = 1;
+  auto fail = [&](hipError_t e, const char *what) {
+    set_error(std::string("femhot_ctx_create: ") + what + " -> " + hipGetErrorString(e));
+    femhot_ctx_destroy(h);
+    return FEMHOT_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&c.d_coords64, sizeof(double) * c.Nn * 3)) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&c.d_coords32, sizeof(float) * c.Nn * 3)) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&c.d_conn, sizeof(int) * c.Ne * c.n)) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&c.d_dmask, c.Ndof)) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMalloc(&c.d_seg_off, sizeof(long long) * (c.nseg + 1))) != hipSuccess) return fail(e, "hipMalloc");
+  if ((e = hipMemcpy(c.d_coords64, c.h_coords.data(), sizeof(double) * c.Nn * 3, hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy");
+  if ((e = hipMemcpy(c.d_coords32, c32.data(), sizeof(float) * c.Nn * 3, hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy");
+  if ((e = hipMemcpy(c.d_conn, c.h_conn.data(), sizeof(int) * c.Ne * c.n, hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e, "hipMemcpy");
+  if ((e = hipMemcpy(c.d_dmask, dmask.data(), c.Ndof, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+  if ((e = hipMemcpy(c.d_seg_off, c.h_seg_off.data(), sizeof(long long) * (c.nseg + 1), hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return fail(e, "hipMemcpy");
+  if ((e = hipHostMalloc((void **)&c.h_pinned, 64 * sizeof(double), hipHostMallocDefault)) != hipSuccess)
+    return fail(e, "hipHostMalloc");
+  *out = h;
+  return FEMHOT_OK;
+}
+
+void femhot_ctx_destroy(femhot_ctx *h) {
+  if (!h) return;
+  Ctx &c = h->c;
+  (void)hipFree(c.d_coords64);
+  (void)hipFree(c.d_coords32);
+  (void)hipFree(c.d_conn);
+  (void)hipFree(c.d_dmask);
+  (void)hipFree(c.d_seg_off);
+  (void)hipFree(c.d_row_ptr);
+  (void)hipFree(c.d_col_idx);
+  (void)hipFree(c.d_slot);
+  (void)hipFree(c.d_colour_elems);
+  free_chunks(c.chunk);
+  (void)hipFree(c.d_ws);
+  if (c.h_pinned) (void)hipHostFree(c.h_pinned);
+  delete h;
+}
+
+int femhot_ctx_info(const femhot_ctx *h, long long *n_dofs, int *npe, int *dpn, long long *nnz, int *n_colours,
+                    int *n_segments) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  Ctx *c = const_cast<Ctx *>(&h->c);
+  if (n_dofs) *n_dofs = c->Ndof;
+  if (npe) *npe = c->n;
+  if (dpn) *dpn = c->ndpn;
+  if (nnz) {
+    int rc = ensure_pattern(c);
+    if (rc) return rc;
+    *nnz = c->nnz;
+  }
+  if (n_colours) {
+    int rc = ensure_colouring(c);
+    if (rc) return rc;
+    *n_colours = c->n_colours;
+  }
+  if (n_segments) *n_segments = c->nseg;
+  return FEMHOT_OK;
+}
+
+int femhot_pattern(femhot_ctx *h, long long *row_ptr, int *col_idx, void *stream) {
+  FH_CHECK_ARG(h && row_ptr && col_idx, FEMHOT_ERR_ARG, "null argument");
+  Ctx *c = &h->c;
+  int rc = ensure_pattern(c);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  FH_HIP(hipMemcpyAsync(row_ptr, c->d_row_ptr, sizeof(long long) * (c->Ndof + 1), hipMemcpyDeviceToDevice, s));
+  if (c->nnz) FH_HIP(hipMemcpyAsync(col_idx, c->d_col_idx, sizeof(int) * c->nnz, hipMemcpyDeviceToDevice, s));
+  return FEMHOT_OK;
+}
+
+int femhot_colouring(femhot_ctx *h, int *colour) {
+  FH_CHECK_ARG(h && colour, FEMHOT_ERR_ARG, "null argument");
+  Ctx *c = &h->c;
+  int rc = ensure_colouring(c);
+  if (rc) return rc;
+  std::memcpy(colour, c->h_colour.data(), sizeof(int) * c->Ne);
+  return FEMHOT_OK;
+}
+
+int femhot_orient_elements(int et, long long n_nodes, const double *coords, long long n_elems, int *conn,
+                           long long *n_swapped) {
+  FH_CHECK_ARG(et >= QUAD4 && et <= TET4, FEMHOT_ERR_ARG, "unknown element type");
+  FH_CHECK_ARG(coords && conn, FEMHOT_ERR_ARG, "null argument");
+  const int n = nodes_per_elem(et);
+  long long sw = 0;
+  double X[8 * 3];
+  for (long long e = 0; e < n_elems; ++e) {
+    for (int a = 0; a < n; ++a) {
+      const int g = conn[e * n + a];
+      FH_CHECK_ARG(g >= 0 && g < n_nodes, FEMHOT_ERR_INDEX, "connectivity index out of range");
+      for (int k = 0; k < 3; ++k) X[a * 3 + k] = coords[(size_t)g * 3 + k];
+    }
+    if (det_at_center(et, X) < 0.0) {
+      std::swap(conn[e * n + 0], conn[e * n + 1]);
+      ++sw;
+    }
+  }
+  if (n_swapped) *n_swapped = sw;
+  return FEMHOT_OK;
+}
+
+}  // extern "C"

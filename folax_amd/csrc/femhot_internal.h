@@ -1,0 +1,100 @@
+// Library-internal context and helpers (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/femhot.h"
+#include "femhot_element.h"
+
+namespace femhot {
+
+void set_error(const std::string &msg);
+const std::string &get_error();
+
+#define FH_CHECK_ARG(cond, code, msg)                                    \
+  do {                                                                   \
+    if (!(cond)) {                                                       \
+      ::femhot::set_error(std::string(__func__) + ": " + (msg));         \
+      return (code);                                                     \
+    }                                                                    \
+  } while (0)
+
+#define FH_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (call);                                                                 \
+    if (_e != hipSuccess) {                                                                 \
+      ::femhot::set_error(std::string(__func__) + ": " #call " -> " + hipGetErrorString(_e)); \
+      return FEMHOT_ERR_HIP;                                                                \
+    }                                                                                       \
+  } while (0)
+
+inline int nodes_per_elem(int et) { return et == HEX8 ? 8 : 4; }
+inline int elem_dim(int et) { return et == QUAD4 ? 2 : 3; }
+
+// Device-resident chunk plan for the LDS-staged scatter (see femhot_chunk.hip).
+struct ChunkPlan {
+  int epc = 0;             // max elements per chunk the plan was built for
+  int n_chunks = 0;
+  int max_nodes = 0;       // max local nodes over chunks
+  int max_elems = 0;
+  long long n_shared = 0;  // nodes touched by more than one chunk
+  int *chunk_elem_off = nullptr;  // (n_chunks+1) offsets into perm / lconn
+  int *chunk_node_off = nullptr;  // (n_chunks+1) offsets into chunk_nodes
+  int *chunk_seg = nullptr;       // (n_chunks) segment id
+  int *perm = nullptr;            // (Ne) original element id at each position
+  uint16_t *lconn = nullptr;      // (Ne * n) chunk-local node indices, position order
+  int *chunk_nodes = nullptr;     // global node id | 0x80000000 when shared
+  int *shared_nodes = nullptr;    // (n_shared) node ids touched by several chunks
+};
+
+struct Ctx {
+  femhot_mesh_desc desc{};
+  int et = 0, ph = 0, ngp = 0, n = 0, dim = 0, ndpn = 0;
+  long long Nn = 0, Ne = 0, Ndof = 0;
+  PhysParams pp{};
+  int nseg = 1;
+  // host copies
+  std::vector<double> h_coords;
+  std::vector<int> h_conn;
+  std::vector<long long> h_dir_idx;
+  std::vector<double> h_dir_val;
+  std::vector<long long> h_seg_off;
+  // device mesh
+  double *d_coords64 = nullptr;  // (Nn*3)
+  float *d_coords32 = nullptr;   // (Nn*3)
+  int *d_conn = nullptr;         // (Ne*n)
+  uint8_t *d_dmask = nullptr;    // (Ndof) 1 = Dirichlet
+  long long *d_seg_off = nullptr;  // (nseg+1)
+  // lazily built caches
+  bool have_pattern = false;
+  std::vector<long long> h_row_ptr;  // (Ndof+1)
+  std::vector<int> h_col_idx;
+  long long nnz = 0;
+  long long *d_row_ptr = nullptr;
+  int *d_col_idx = nullptr;
+  uint8_t *d_slot = nullptr;  // (Ne*n*n) position of node b in node a's sorted neighbour list
+  bool have_colour = false;
+  std::vector<int> h_colour;
+  int n_colours = 0;
+  std::vector<int> h_colour_off;  // (n_colours+1)
+  int *d_colour_elems = nullptr;  // elements sorted by colour (stable)
+  ChunkPlan chunk;                // built on demand
+  // solver workspace
+  size_t ws_bytes = 0;
+  void *d_ws = nullptr;
+  double *h_pinned = nullptr;  // small pinned host scalars
+};
+
+int ensure_pattern(Ctx *c);
+int ensure_colouring(Ctx *c);
+int ensure_chunks(Ctx *c, int epc);
+int ensure_workspace(Ctx *c, size_t bytes);
+
+}  // namespace femhot
+
+struct femhot_ctx {
+  femhot::Ctx c;
+};

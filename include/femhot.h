@@ -1,0 +1,156 @@
+/*
+ * femhot -- MI355X (gfx950) finite-element residual hot path, C ABI.
+ *
+ * Plain C interface: opaque context handle, raw device pointers, element counts,
+ * hipStream_t passed as void*.  No torch / XLA types cross this boundary.  Every
+ * call is stream-ordered on the caller's stream; the library allocates device
+ * memory only in femhot_ctx_create / femhot_ctx_destroy and the lazily built
+ * context caches (pattern, slot table, colouring, chunk plan, solver workspace),
+ * which are created on the first call that needs them.
+ *
+ * Reference interfaces replaced (folax, paths relative to the repository root):
+ *  - fol/loss_functions/fe_loss.py:57-122   FiniteElementLoss.Initialize  -> femhot_ctx_create
+ *  - fol/loss_functions/fe_loss.py:166-176  ComputeElementsEnergies / ComputeTotalEnergy
+ *                                           -> femhot_residual (energy output)
+ *  - fol/loss_functions/fe_loss.py:250-262  ComputeBatchLoss (per-sample energies, B >= 1)
+ *                                           -> femhot_residual / femhot_energy_grad with batch
+ *  - fol/loss_functions/fe_loss.py:264-318  ComputeJacobianMatrixAndResidualVector
+ *                                           -> femhot_pattern + femhot_tangent_csr
+ *  - fol/loss_functions/ffi_functions/kr_small_displacement_element.cc:178-224,226-291
+ *      (XLA FFI handlers compute_nodal_residuals / compute_elements)
+ *                                           -> femhot_residual / femhot_tangent_csr
+ *  - fol/responses/fe_response.py:434-526   ComputeLossElementControlGrad / adjoint nodal control
+ *                                           derivatives (lambda^T dR/dk) -> femhot_residual_vjp
+ *  - jax.grad of fe_loss.py:175 ComputeTotalEnergy (VJP of the energy) -> femhot_energy_grad
+ *  - fol/solvers/fe_solver.py:60-78         LinearSolve (BiCGSTAB / direct on the assembled
+ *                                           tangent) -> femhot_pcg (Jacobi-PCG, FP64 or FP32
+ *                                           inner iterations with FP64 refinement)
+ *  - fol/mesh_input_output/mesh.py:123-142  CheckAndOrientElements -> femhot_orient_elements
+ */
+#ifndef FEMHOT_H
+#define FEMHOT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types (fol/geometries/__init__.py:18-21 names: "quad", "hexahedron", "tetra") */
+#define FEMHOT_QUAD4 0
+#define FEMHOT_HEX8 1
+#define FEMHOT_TET4 2
+
+/* physics (ThermalLoss thermal.py:15, MechanicalLoss mechanical.py:11,
+ * NeoHookeMechanicalLoss mechanical_neohooke.py:16) */
+#define FEMHOT_THERMAL 0
+#define FEMHOT_MECHANICAL 1
+#define FEMHOT_NEOHOOKE 2
+
+/* arithmetic type of a call */
+#define FEMHOT_F32 0
+#define FEMHOT_F64 1
+
+/* call flags */
+#define FEMHOT_MASK_DIRICHLET 0x1u  /* zero Dirichlet dofs of residual / g_u (fe_loss.py:191-207) */
+#define FEMHOT_ELEMENT_FIELD 0x2u   /* material/control field given per element, not per node */
+#define FEMHOT_PICARD 0x4u          /* thermal VJP with k(T) frozen: thermal.py:31 stop_gradient */
+#define FEMHOT_TRANSPOSE 0x8u       /* tangent: transpose element matrices before masking (:219-228) */
+#define FEMHOT_ACCUMULATE 0x10u     /* add into outputs instead of overwriting them */
+#define FEMHOT_SCATTER_COLOUR 0x20u /* colour-ordered, non-atomic, deterministic scatter */
+#define FEMHOT_SCATTER_CHUNK 0x40u  /* LDS-staged chunk scatter (persistent launch) */
+
+/* status codes */
+#define FEMHOT_OK 0
+#define FEMHOT_ERR_ARG 1          /* bad argument (null pointer, unsupported enum, ...) */
+#define FEMHOT_ERR_SHAPE 2        /* sizes inconsistent */
+#define FEMHOT_ERR_INDEX 3        /* connectivity / dof index out of range */
+#define FEMHOT_ERR_NONFINITE 4    /* non-finite input where finite is required */
+#define FEMHOT_ERR_HIP 5          /* HIP runtime error (message has the HIP error string) */
+#define FEMHOT_ERR_NOT_CONVERGED 6
+#define FEMHOT_ERR_UNSUPPORTED 7  /* combination not implemented */
+
+typedef struct femhot_ctx femhot_ctx;
+
+/* Mesh + loss settings.  All pointers are HOST pointers, read during create only. */
+typedef struct {
+  int elem_type;       /* FEMHOT_QUAD4 / FEMHOT_HEX8 / FEMHOT_TET4 */
+  int physics;         /* FEMHOT_THERMAL / FEMHOT_MECHANICAL / FEMHOT_NEOHOOKE */
+  int num_gp;          /* Gauss rule 1/2/3 (fe_loss.py:71-84); 0 = the reference default */
+  double young_modulus;   /* material_dict E (mechanical: scales D; unused by neohooke) */
+  double poisson_ratio;   /* material_dict nu */
+  double beta, c;         /* thermal k(T) = k0 (1 + beta T^c) (thermal.py:22-26) */
+  double body_force[3];   /* mechanical body_foce / thermal scalar source in [0] */
+  long long n_nodes;
+  long long n_elems;
+  const double *coords;   /* (n_nodes, 3) row-major */
+  const int *conn;        /* (n_elems, nodes_per_elem) row-major, int32 */
+  long long n_dirichlet;
+  const long long *dirichlet_idx; /* global dof ids ndpn*node+comp (fe_loss.py:34-55) */
+  const double *dirichlet_val;
+  int n_segments;                  /* >1: disjoint meshes, energies reported per segment */
+  const long long *segment_offsets; /* (n_segments+1) element offsets, or NULL */
+} femhot_mesh_desc;
+
+const char *femhot_last_error(void);
+const char *femhot_version(void);
+
+int femhot_ctx_create(const femhot_mesh_desc *desc, femhot_ctx **out);
+void femhot_ctx_destroy(femhot_ctx *ctx);
+
+/* sizes: dofs, nodes per element, dofs per node, nnz of the tangent, colours, segments */
+int femhot_ctx_info(const femhot_ctx *ctx, long long *n_dofs, int *nodes_per_elem, int *dofs_per_node,
+                    long long *nnz, int *n_colours, int *n_segments);
+
+/* CSR pattern of the tangent (scipy csr_array of the reference COO: sorted columns, summed
+ * duplicates, explicit zeros kept).  row_ptr: (n_dofs+1) int64, col_idx: (nnz) int32, DEVICE. */
+int femhot_pattern(femhot_ctx *ctx, long long *row_ptr, int *col_idx, void *stream);
+
+/* greedy first-fit element colouring in element order (HOST output, n_elems int32). */
+int femhot_colouring(femhot_ctx *ctx, int *colour);
+
+/* Residual + element energies.  field: (batch, n_nodes) or (batch, n_elems) with
+ * FEMHOT_ELEMENT_FIELD; u: (batch, n_dofs); R: (batch, n_dofs) or NULL; energy: double,
+ * (batch * max(1,n_segments)) or NULL -- the reference element energies summed
+ * (thermal T.r, mechanical u.r, neohooke Ee).  Strides in elements of the arithmetic type. */
+int femhot_residual(femhot_ctx *ctx, int dtype, int batch, const void *field, long long field_stride,
+                    const void *u, long long u_stride, void *R, long long R_stride, double *energy,
+                    unsigned flags, void *stream);
+
+/* Tangent values on the CSR pattern (+ optional residual R), BC-masked exactly like
+ * fe_loss.py:191-207; FEMHOT_TRANSPOSE transposes each element matrix first. */
+int femhot_tangent_csr(femhot_ctx *ctx, int dtype, const void *field, const void *u, void *vals, void *R,
+                       unsigned flags, void *stream);
+
+/* VJP of the residual: g_u = (dR/du)^T w, g_field = (dR/dfield)^T w (exact; FEMHOT_PICARD
+ * freezes k(T) for thermal).  Outputs are overwritten unless FEMHOT_ACCUMULATE. */
+int femhot_residual_vjp(femhot_ctx *ctx, int dtype, int batch, const void *field, long long field_stride,
+                        const void *u, long long u_stride, const void *w, long long w_stride, void *g_u,
+                        long long g_u_stride, void *g_field, long long g_field_stride, unsigned flags,
+                        void *stream);
+
+/* Potential Pi and its gradient: g_u = dPi/du (= residual), g_field = dPi/dfield.
+ * energy: double (batch * max(1,n_segments)) or NULL. */
+int femhot_energy_grad(femhot_ctx *ctx, int dtype, int batch, const void *field, long long field_stride,
+                       const void *u, long long u_stride, double *energy, void *g_u, long long g_u_stride,
+                       void *g_field, long long g_field_stride, unsigned flags, void *stream);
+
+/* Jacobi-preconditioned CG on the CSR tangent for K x = b (Dirichlet rows of b must be 0).
+ * mode FEMHOT_F64: FP64 PCG.  mode FEMHOT_F32: FP32 matrix/vectors inside, FP64 dot products,
+ * FP64 outer iterative refinement with the FP64 matrix.  x: in = initial guess, out = solution.
+ * Returns FEMHOT_ERR_NOT_CONVERGED (x still written) when maxiter is reached. */
+int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *col_idx, const double *vals,
+               const double *b, double *x, double rel_tol, int maxiter, int *iters, double *rel_res,
+               void *stream);
+
+/* CSR SpMV y = A x (FP64), used by the solver and exposed for tests. */
+int femhot_spmv(femhot_ctx *ctx, const long long *row_ptr, const int *col_idx, const double *vals,
+                const double *x, double *y, void *stream);
+
+/* Orientation fix (mesh.py:123-142) on a HOST connectivity array, in place: where detJ at the
+ * one-point rule is negative, swap local nodes 0 and 1.  Returns the number swapped in *n_swapped. */
+int femhot_orient_elements(int elem_type, long long n_nodes, const double *coords, long long n_elems, int *conn,
+                           long long *n_swapped);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEMHOT_H */
