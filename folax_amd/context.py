@@ -182,6 +182,38 @@ class FemhotContext:
                                              _ptr(gu), self.n_dofs, _ptr(gd), fsz, flags, _stream()))
         return (en.view(B, self.n_segments) if en is not None else None), gu, gd
 
+    def start_kernel_timing(self):
+        L.check(self._lib.femhot_kernel_timing(self._h, 1, None, None))
+
+    def stop_kernel_timing(self):
+        """(total ms, number of timed calls) of the dominant kernels since start_kernel_timing()."""
+        ms, n = C.c_double(), C.c_longlong()
+        L.check(self._lib.femhot_kernel_timing(self._h, 0, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def autotune(self, op, field, u, w=None, flags=0):
+        """Pick elements-per-chunk / workgroup size / grid shape for the chunk path by timing them on
+        these inputs (L.OP_RESIDUAL / OP_VJP / OP_ENERGY_GRAD).  Returns (epc, block, grid_per_cu, ms)."""
+        u = u if u.dim() == 2 else u.unsqueeze(0)
+        field = field if field.dim() == 2 else field.unsqueeze(0)
+        B = u.shape[0]
+        dt, dev = u.dtype, u.device
+        us = self._check_in(u, "u", self.n_dofs, B, dt, dev)
+        fsz = self._field_size(flags)
+        fs = self._check_in(field, "field", fsz, B, dt, dev)
+        ws = 0
+        if w is not None:
+            w = w if w.dim() == 2 else w.unsqueeze(0)
+            ws = self._check_in(w, "w", self.n_dofs, B, dt, dev)
+        ou = torch.empty(B, self.n_dofs, dtype=dt, device=dev)
+        od = torch.empty(B, fsz, dtype=dt, device=dev) if op != L.OP_RESIDUAL else None
+        en = torch.empty(B * self.n_segments, dtype=torch.float64, device=dev)
+        epc, blk, grid, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        L.check(self._lib.femhot_autotune(self._h, int(op), _dtype_code(u), B, _ptr(field), fs, _ptr(u), us, _ptr(w),
+                                          ws, _ptr(ou), self.n_dofs, _ptr(od), fsz, _ptr(en), flags, C.byref(epc),
+                                          C.byref(blk), C.byref(grid), C.byref(ms), _stream()))
+        return epc.value, blk.value, grid.value, ms.value
+
     def tangent_csr(self, field, u, want_R=True, flags=0):
         """CSR values (nnz) on the pattern of `pattern()`, BC-masked, and the masked residual."""
         field = field.reshape(-1)

@@ -62,8 +62,9 @@ static int run_elem(Ctx *c, const CallIO &io) {
   const long long fsize = elemental ? c->Ne : c->Nn;
   const bool acc = (io.flags & FEMHOT_ACCUMULATE) != 0;
   int rc;
-  // clear outputs
-  if (!acc) {
+  const bool chunk = (io.flags & FEMHOT_SCATTER_CHUNK) != 0;
+  // clear outputs (the chunk path writes every node itself and clears only shared nodes)
+  if (!acc && !chunk) {
     if (io.dtype == FEMHOT_F32) {
       if ((rc = fill<float>((float *)io.ou, c->Ndof, io.batch > 1 ? io.ous : c->Ndof, io.batch, 0.f, io.stream)))
         return rc;
@@ -77,11 +78,11 @@ static int run_elem(Ctx *c, const CallIO &io) {
         if ((rc = fill<double>((double *)io.od, fsize, io.batch > 1 ? io.ods : fsize, io.batch, 0.0, io.stream)))
           return rc;
     }
-    if ((rc = fill<double>(io.energy, (long long)io.batch * c->nseg, (long long)io.batch * c->nseg, 1, 0.0,
-                           io.stream)))
-      return rc;
   }
-  if (io.flags & FEMHOT_SCATTER_CHUNK) {
+  if (!acc && (rc = fill<double>(io.energy, (long long)io.batch * c->nseg, (long long)io.batch * c->nseg, 1, 0.0,
+                                 io.stream)))
+    return rc;
+  if (chunk) {
     return run_chunk(c, io.dtype, io.mode, io.batch, io.field, io.fs, io.u, io.us, io.w, io.ws, io.ou, io.ous, io.od,
                      io.ods, io.energy, io.flags, io.stream, 0, 0);
   }
@@ -112,6 +113,7 @@ static int run_elem(Ctx *c, const CallIO &io) {
   A.nseg = c->nseg;
   A.flags = io.flags;
   A.pp = c->pp;
+  if ((rc = timing_mark(c, io.stream))) return rc;
   if (io.flags & FEMHOT_SCATTER_COLOUR) {
     if ((rc = ensure_colouring(c))) return rc;
     L.plain = true;
@@ -123,13 +125,14 @@ static int run_elem(Ctx *c, const CallIO &io) {
       L.grid = dim3((unsigned)((cnt + 255) / 256), io.batch);
       if ((rc = launch_elem_any(c->et, L))) return rc;
     }
-    return FEMHOT_OK;
+    return timing_mark(c, io.stream);
   }
   L.plain = false;
   A.list = nullptr;
   A.count = c->Ne;
   L.grid = dim3((unsigned)((c->Ne + 255) / 256), io.batch);
-  return launch_elem_any(c->et, L);
+  if ((rc = launch_elem_any(c->et, L))) return rc;
+  return timing_mark(c, io.stream);
 }
 
 }  // namespace femhot
@@ -274,6 +277,40 @@ int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *
     if ((rc = run_elem(c, io))) return rc;
   }
   return FEMHOT_OK;
+}
+
+int femhot_autotune(femhot_ctx *h, int op, int dtype, int batch, const void *field, long long fs, const void *u,
+                    long long us, const void *w, long long ws, void *out_u, long long ous, void *out_field,
+                    long long ofs, double *energy, unsigned flags, int *epc, int *block, int *grid_per_cu,
+                    float *ms, void *stream) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  FH_CHECK_ARG(op >= FEMHOT_OP_RESIDUAL && op <= FEMHOT_OP_ENERGY_GRAD, FEMHOT_ERR_ARG, "femhot_autotune: bad op");
+  FH_CHECK_ARG(op != FEMHOT_OP_VJP || w, FEMHOT_ERR_ARG, "femhot_autotune: VJP needs a cotangent");
+  Ctx *c = &h->c;
+  CallIO io;
+  io.dtype = dtype;
+  io.batch = batch;
+  io.mode = op;  // FEMHOT_OP_* equal MODE_*
+  io.field = field;
+  io.fs = fs;
+  io.u = u;
+  io.us = us;
+  io.w = w;
+  io.ws = ws;
+  io.ou = out_u;
+  io.ous = ous;
+  io.od = out_field;
+  io.ods = ofs;
+  io.energy = energy;
+  io.flags = (flags & ~FEMHOT_ACCUMULATE) | FEMHOT_SCATTER_CHUNK;
+  io.stream = (hipStream_t)stream;
+  int rc = validate(c, io, "femhot_autotune");
+  if (rc) return rc;
+  rc = autotune_chunk(c, dtype, op, batch, field, io.batch > 1 ? fs : 0, u, io.batch > 1 ? us : 0, w,
+                      io.batch > 1 ? ws : 0, out_u, ous, out_field, ofs, energy, io.flags, io.stream, epc, block,
+                      grid_per_cu, ms);
+  if (rc) return rc;
+  return run_elem(c, io);  // leave valid outputs
 }
 
 }  // extern "C"

@@ -10,4 +10,14 @@ int run_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, long lo
               const void *w, long long ws, void *ou, long long ous, void *od, long long ods, double *energy,
               unsigned flags, hipStream_t stream, int epc, int block);
 
+// Times chunk configurations (elements per chunk x workgroup size x grid shape) on the given
+// inputs with HIP events and records the fastest for (ctx, mode, dtype, batch, field location).
+// Outputs are scratch while tuning.  Synchronises the stream; not for use under graph capture.
+int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, long long fs, const void *u,
+                   long long us, const void *w, long long ws, void *ou, long long ous, void *od, long long ods,
+                   double *energy, unsigned flags, hipStream_t s, int *best_epc, int *best_block, int *best_grid,
+                   float *best_ms);
+
+void forget_tuning(const void *ctx);
+
 }  // namespace femhot

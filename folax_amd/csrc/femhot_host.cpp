@@ -13,6 +13,7 @@
 #include <cstring>
 #include <numeric>
 
+#include "femhot_chunk.h"
 #include "femhot_internal.h"
 
 namespace femhot {
@@ -159,13 +160,24 @@ static void free_chunks(ChunkPlan &p) {
   (void)hipFree(p.perm);
   (void)hipFree(p.lconn);
   (void)hipFree(p.chunk_nodes);
-  (void)hipFree(p.shared_nodes);
+  (void)hipFree(p.chunk_xyz32);
+  (void)hipFree(p.chunk_xyz64);
+  (void)hipFree(p.zero_nodes);
   p = ChunkPlan{};
 }
 
-// Chunk plan: elements ordered by (segment, Morton code of the centroid) and cut into runs of at
-// most `epc` elements; each chunk lists its nodes (sorted), its elements' chunk-local connectivity
-// and which of its nodes are shared with other chunks.
+template <class T> static int upload(T **dst, const std::vector<T> &src) {
+  FH_HIP(hipMalloc(dst, sizeof(T) * std::max<size_t>(1, src.size())));
+  if (!src.empty()) FH_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return FEMHOT_OK;
+}
+
+// Chunk plan for the LDS-staged scatter.  Elements are ordered by (segment, Morton code of the
+// centroid) and cut into runs of at most `epc` elements that never cross a segment; inside a chunk
+// the elements keep their original relative order (so per-element inputs/outputs are read and
+// written in contiguous runs).  Each chunk lists its nodes sorted by global id, flags the nodes
+// shared with other chunks, stores its elements' chunk-local connectivity (uint16) and the
+// coordinates of its nodes relative to its first node (FP32 and FP64).
 int ensure_chunks(Ctx *c, int epc) {
   if (c->chunk.n_chunks && c->chunk.epc == epc) return FEMHOT_OK;
   if (epc < 1 || epc > 8192) {
@@ -204,6 +216,8 @@ int ensure_chunks(Ctx *c, int epc) {
   });
   std::vector<int> perm(Ne), elem_off{0}, node_off{0}, seg, nodes;
   std::vector<uint16_t> lconn((size_t)Ne * n);
+  std::vector<float> xyz32;
+  std::vector<double> xyz64;
   for (long long i = 0; i < Ne; ++i) perm[i] = key[i].second;
   long long i = 0;
   std::vector<int> loc;
@@ -212,15 +226,27 @@ int ensure_chunks(Ctx *c, int epc) {
     long long j = i;
     const int s = seg_of[perm[i]];
     while (j < Ne && j - i < epc && seg_of[perm[j]] == s) ++j;
+    std::sort(perm.begin() + i, perm.begin() + j);
     loc.clear();
     for (long long k = i; k < j; ++k)
       for (int a = 0; a < n; ++a) loc.push_back(c->h_conn[(size_t)perm[k] * n + a]);
     std::sort(loc.begin(), loc.end());
     loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+    if (loc.size() > 65535) {
+      set_error("ensure_chunks: more than 65535 nodes in a chunk");
+      return FEMHOT_ERR_UNSUPPORTED;
+    }
     for (long long k = i; k < j; ++k)
       for (int a = 0; a < n; ++a) {
         const int g = c->h_conn[(size_t)perm[k] * n + a];
         lconn[(size_t)k * n + a] = (uint16_t)(std::lower_bound(loc.begin(), loc.end(), g) - loc.begin());
+      }
+    const double *ref = &c->h_coords[(size_t)loc[0] * 3];
+    for (int g : loc)
+      for (int k = 0; k < 3; ++k) {
+        const double v = c->h_coords[(size_t)g * 3 + k] - ref[k];
+        xyz64.push_back(v);
+        xyz32.push_back((float)v);
       }
     nodes.insert(nodes.end(), loc.begin(), loc.end());
     max_nodes = std::max<int>(max_nodes, (int)loc.size());
@@ -232,9 +258,12 @@ int ensure_chunks(Ctx *c, int epc) {
   }
   std::vector<int> touch(c->Nn, 0);
   for (int g : nodes) touch[g]++;
-  std::vector<int> shared;
-  for (long long g = 0; g < c->Nn; ++g)
-    if (touch[g] > 1) shared.push_back((int)g);
+  std::vector<int> zero;
+  long long n_shared = 0;
+  for (long long g = 0; g < c->Nn; ++g) {
+    if (touch[g] != 1) zero.push_back((int)g);
+    n_shared += touch[g] > 1;
+  }
   for (int &g : nodes)
     if (touch[g] > 1) g = (int)((unsigned)g | 0x80000000u);
   ChunkPlan &p = c->chunk;
@@ -242,24 +271,27 @@ int ensure_chunks(Ctx *c, int epc) {
   p.n_chunks = (int)seg.size();
   p.max_nodes = max_nodes;
   p.max_elems = max_elems;
-  p.n_shared = (long long)shared.size();
-  FH_HIP(hipMalloc(&p.chunk_elem_off, sizeof(int) * elem_off.size()));
-  FH_HIP(hipMalloc(&p.chunk_node_off, sizeof(int) * node_off.size()));
-  FH_HIP(hipMalloc(&p.chunk_seg, sizeof(int) * std::max<size_t>(1, seg.size())));
-  FH_HIP(hipMalloc(&p.perm, sizeof(int) * std::max<long long>(1, Ne)));
-  FH_HIP(hipMalloc(&p.lconn, sizeof(uint16_t) * std::max<size_t>(1, lconn.size())));
-  FH_HIP(hipMalloc(&p.chunk_nodes, sizeof(int) * std::max<size_t>(1, nodes.size())));
-  FH_HIP(hipMalloc(&p.shared_nodes, sizeof(int) * std::max<size_t>(1, shared.size())));
-  FH_HIP(hipMemcpy(p.chunk_elem_off, elem_off.data(), sizeof(int) * elem_off.size(), hipMemcpyHostToDevice));
-  FH_HIP(hipMemcpy(p.chunk_node_off, node_off.data(), sizeof(int) * node_off.size(), hipMemcpyHostToDevice));
-  if (!seg.empty()) FH_HIP(hipMemcpy(p.chunk_seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice));
-  if (Ne) FH_HIP(hipMemcpy(p.perm, perm.data(), sizeof(int) * Ne, hipMemcpyHostToDevice));
-  if (!lconn.empty())
-    FH_HIP(hipMemcpy(p.lconn, lconn.data(), sizeof(uint16_t) * lconn.size(), hipMemcpyHostToDevice));
-  if (!nodes.empty())
-    FH_HIP(hipMemcpy(p.chunk_nodes, nodes.data(), sizeof(int) * nodes.size(), hipMemcpyHostToDevice));
-  if (!shared.empty())
-    FH_HIP(hipMemcpy(p.shared_nodes, shared.data(), sizeof(int) * shared.size(), hipMemcpyHostToDevice));
+  p.n_shared = n_shared;
+  p.n_zero = (long long)zero.size();
+  int rc;
+  if ((rc = upload(&p.chunk_elem_off, elem_off)) || (rc = upload(&p.chunk_node_off, node_off)) ||
+      (rc = upload(&p.chunk_seg, seg)) || (rc = upload(&p.perm, perm)) || (rc = upload(&p.lconn, lconn)) ||
+      (rc = upload(&p.chunk_nodes, nodes)) || (rc = upload(&p.chunk_xyz32, xyz32)) ||
+      (rc = upload(&p.chunk_xyz64, xyz64)) || (rc = upload(&p.zero_nodes, zero))) {
+    free_chunks(p);
+    return rc;
+  }
+  return FEMHOT_OK;
+}
+
+int timing_mark(Ctx *c, hipStream_t s) {
+  if (!c->timing) return FEMHOT_OK;
+  if (c->tev_used == c->tev_pool.size()) {
+    hipEvent_t e;
+    FH_HIP(hipEventCreate(&e));
+    c->tev_pool.push_back(e);
+  }
+  FH_HIP(hipEventRecord(c->tev_pool[c->tev_used++], s));
   return FEMHOT_OK;
 }
 
@@ -430,6 +462,7 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
 
 void femhot_ctx_destroy(femhot_ctx *h) {
   if (!h) return;
+  forget_tuning(&h->c);
   Ctx &c = h->c;
   (void)hipFree(c.d_coords64);
   (void)hipFree(c.d_coords32);
@@ -442,6 +475,7 @@ void femhot_ctx_destroy(femhot_ctx *h) {
   (void)hipFree(c.d_colour_elems);
   free_chunks(c.chunk);
   (void)hipFree(c.d_ws);
+  for (hipEvent_t e : c.tev_pool) (void)hipEventDestroy(e);
   if (c.h_pinned) (void)hipHostFree(c.h_pinned);
   delete h;
 }
@@ -484,6 +518,30 @@ int femhot_colouring(femhot_ctx *h, int *colour) {
   int rc = ensure_colouring(c);
   if (rc) return rc;
   std::memcpy(colour, c->h_colour.data(), sizeof(int) * c->Ne);
+  return FEMHOT_OK;
+}
+
+int femhot_kernel_timing(femhot_ctx *h, int enable, double *total_ms, long long *launches) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  Ctx &c = h->c;
+  if (enable) {
+    c.timing = true;
+    c.tev_used = 0;
+    return FEMHOT_OK;
+  }
+  double ms = 0.0;
+  long long n = 0;
+  for (size_t i = 0; i + 1 < c.tev_used; i += 2) {
+    FH_HIP(hipEventSynchronize(c.tev_pool[i + 1]));
+    float t = 0.f;
+    FH_HIP(hipEventElapsedTime(&t, c.tev_pool[i], c.tev_pool[i + 1]));
+    ms += t;
+    ++n;
+  }
+  c.timing = false;
+  c.tev_used = 0;
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = n;
   return FEMHOT_OK;
 }
 

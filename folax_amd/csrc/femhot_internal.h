@@ -47,7 +47,10 @@ struct ChunkPlan {
   int *perm = nullptr;            // (Ne) original element id at each position
   uint16_t *lconn = nullptr;      // (Ne * n) chunk-local node indices, position order
   int *chunk_nodes = nullptr;     // global node id | 0x80000000 when shared
-  int *shared_nodes = nullptr;    // (n_shared) node ids touched by several chunks
+  float *chunk_xyz32 = nullptr;   // per chunk node: coordinates relative to the chunk's first node
+  double *chunk_xyz64 = nullptr;
+  long long n_zero = 0;           // nodes whose outputs must be cleared before a chunk launch
+  int *zero_nodes = nullptr;      // shared nodes + nodes in no element
 };
 
 struct Ctx {
@@ -82,6 +85,10 @@ struct Ctx {
   std::vector<int> h_colour_off;  // (n_colours+1)
   int *d_colour_elems = nullptr;  // elements sorted by colour (stable)
   ChunkPlan chunk;                // built on demand
+  // kernel timing hook: events recorded around the dominant kernel of each call while enabled
+  bool timing = false;
+  std::vector<hipEvent_t> tev_pool;
+  size_t tev_used = 0;
   // solver workspace
   size_t ws_bytes = 0;
   void *d_ws = nullptr;
@@ -92,6 +99,9 @@ int ensure_pattern(Ctx *c);
 int ensure_colouring(Ctx *c);
 int ensure_chunks(Ctx *c, int epc);
 int ensure_workspace(Ctx *c, size_t bytes);
+
+// Records an event on `s` when timing is enabled (pairs: before/after the dominant kernel).
+int timing_mark(Ctx *c, hipStream_t s);
 
 }  // namespace femhot
 

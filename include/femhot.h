@@ -141,6 +141,26 @@ int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *c
                const double *b, double *x, double rel_tol, int maxiter, int *iters, double *rel_res,
                void *stream);
 
+/* Autotuner for the LDS-staged chunk path (FEMHOT_SCATTER_CHUNK): times elements-per-chunk x
+ * workgroup size x grid shape (one workgroup per chunk, or a persistent grid of k workgroups per
+ * CU) on these inputs and keeps the fastest for (ctx, op, dtype, batch, field location); then runs
+ * the operation once more so the outputs are valid.  Synchronises the stream (not for capture).
+ * op: FEMHOT_OP_RESIDUAL / FEMHOT_OP_VJP / FEMHOT_OP_ENERGY_GRAD.  No counterpart in the
+ * reference (XLA fuses the element vmap itself); it replaces fe_loss.py:105-114's fixed batch size. */
+#define FEMHOT_OP_RESIDUAL 0
+#define FEMHOT_OP_VJP 1
+#define FEMHOT_OP_ENERGY_GRAD 2
+int femhot_autotune(femhot_ctx *ctx, int op, int dtype, int batch, const void *field, long long field_stride,
+                    const void *u, long long u_stride, const void *w, long long w_stride, void *out_u,
+                    long long out_u_stride, void *out_field, long long out_field_stride, double *energy,
+                    unsigned flags, int *epc, int *block, int *grid_per_cu, float *ms, void *stream);
+
+/* Measurement hook (no counterpart in the reference): enable=1 starts recording HIP events on the
+ * call's stream immediately before and after the dominant element kernel(s) of every residual /
+ * VJP / potential-gradient call on this context; enable=0 waits for them, returns the summed
+ * kernel time and the number of timed calls, and stops recording.  Not for use under capture. */
+int femhot_kernel_timing(femhot_ctx *ctx, int enable, double *total_ms, long long *launches);
+
 /* CSR SpMV y = A x (FP64), used by the solver and exposed for tests. */
 int femhot_spmv(femhot_ctx *ctx, const long long *row_ptr, const int *col_idx, const double *vals,
                 const double *x, double *y, void *stream);

@@ -259,3 +259,19 @@ def test_planted_missing_exact_term_is_caught(monkeypatch):
     eps = 1e-6
     fd = case["w"] @ (_R(case, case["u"] + eps * v, case["field"]) - _R(case, case["u"] - eps * v, case["field"])) / (2 * eps)
     assert abs(fd - gu @ v) > 1e-4 * abs(fd)
+
+
+def test_c_restatement_matches_numpy_oracle():
+    """oracle/cpu_ref.c (the timed CPU baseline) equals the NumPy oracle on a jittered TET4 mesh."""
+    from oracle import cpuref
+    coords, conn, _ = fe.box_tet_mesh(4, 1.0, jitter=0.2, seed=9)
+    rng = np.random.default_rng(10)
+    E = np.exp(rng.uniform(0, np.log(100.0), size=conn.shape[0]))
+    u = rng.uniform(-1e-3, 1e-3, size=3 * coords.shape[0])
+    seg = (np.arange(conn.shape[0]) >= conn.shape[0] // 2).astype(np.int64)
+    Pi, R, gE = cpuref.tet4_elastic_energy_grad(coords, conn, E, u, seg_of_elem=seg, n_seg=2)
+    ph = fe.Physics("mechanical", "tetra", E=1.0, nu=0.3)
+    Pe, r, gde = fe.energy_grad(ph, coords[conn], E, u[fe.element_dofs(conn, 3)])
+    np.testing.assert_allclose(R, fe.scatter_nodal(conn, r, coords.shape[0], 3), rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(gE, gde, rtol=1e-12)
+    np.testing.assert_allclose(Pi, [Pe[seg == 0].sum(), Pe[seg == 1].sum()], rtol=1e-12)
