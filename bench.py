@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (FP32 chunk path only)")
+    ap.add_argument("--epc", type=int, default=0, help="fixed elements per chunk (skips the autotuner)")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--grid", type=int, default=0, help="workgroups per CU (0: one per chunk)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build has no multi-GPU path)",
@@ -121,9 +124,13 @@ def main():
     gd32 = torch.empty(1, Ne, dtype=torch.float32, device=dev)
     fl32 = flags | L.SCATTER_CHUNK
     t1 = time.time()
-    epc, blk, grid, tms = ctx.autotune(L.OP_ENERGY_GRAD, E32, u32, flags=flags)
-    log(f"autotune: {epc} elements/chunk, {blk} threads, grid {'all chunks' if grid == 0 else f'{grid}/CU'}, "
-        f"{tms * 1e3:.1f} us ({time.time() - t1:.1f}s)")
+    if args.epc:
+        epc, blk, grid, tms = args.epc, args.block, args.grid, float("nan")
+        ctx.set_chunk_config(L.OP_ENERGY_GRAD, torch.float32, 1, epc, blk, grid, flags=flags)
+    else:
+        epc, blk, grid, tms = ctx.autotune(L.OP_ENERGY_GRAD, E32, u32, flags=flags)
+    log(f"chunk config: {epc} elements/chunk, {blk} threads, grid {'all chunks' if grid == 0 else f'{grid}/CU'}, "
+        f"tuned {tms * 1e3:.1f} us ({time.time() - t1:.1f}s)")
 
     def step32():
         ctx.energy_grad(E32, u32, flags=fl32, out=(en32, gu32, gd32))
@@ -135,6 +142,11 @@ def main():
     if args.profile:
         for _ in range(args.steps):
             step32()
+        # calibration for HBM counters: a plain copy of 256 MiB (read 256 MiB, write 256 MiB)
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
         torch.cuda.synchronize()
         log("profile run done")
         return

@@ -33,7 +33,7 @@ EXPORTED = [
     "femhot_last_error", "femhot_version", "femhot_ctx_create", "femhot_ctx_destroy", "femhot_ctx_info",
     "femhot_pattern", "femhot_colouring", "femhot_residual", "femhot_tangent_csr", "femhot_residual_vjp",
     "femhot_energy_grad", "femhot_pcg", "femhot_spmv", "femhot_orient_elements", "femhot_autotune",
-    "femhot_kernel_timing",
+    "femhot_kernel_timing", "femhot_set_chunk_config",
 ]
 
 OP_RESIDUAL, OP_VJP, OP_ENERGY_GRAD = 0, 1, 2
@@ -89,6 +89,7 @@ def lib():
                                    C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_float), vp]
     L.femhot_orient_elements.argtypes = [i32, ll, C.POINTER(dbl), ll, C.POINTER(i32), C.POINTER(ll)]
     L.femhot_kernel_timing.argtypes = [vp, i32, C.POINTER(dbl), C.POINTER(ll)]
+    L.femhot_set_chunk_config.argtypes = [vp, i32, i32, i32, u32, i32, i32, i32]
     for name in EXPORTED:
         if name not in ("femhot_last_error", "femhot_version", "femhot_ctx_destroy"):
             getattr(L, name).restype = C.c_int

@@ -182,6 +182,11 @@ class FemhotContext:
                                              _ptr(gu), self.n_dofs, _ptr(gd), fsz, flags, _stream()))
         return (en.view(B, self.n_segments) if en is not None else None), gu, gd
 
+    def set_chunk_config(self, op, dtype, batch, epc, block, grid_per_cu=0, flags=0):
+        code = L.F32 if dtype == torch.float32 else L.F64
+        L.check(self._lib.femhot_set_chunk_config(self._h, int(op), code, int(batch), int(flags), int(epc),
+                                                  int(block), int(grid_per_cu)))
+
     def start_kernel_timing(self):
         L.check(self._lib.femhot_kernel_timing(self._h, 1, None, None))
 

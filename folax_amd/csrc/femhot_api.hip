@@ -79,7 +79,7 @@ static int run_elem(Ctx *c, const CallIO &io) {
           return rc;
     }
   }
-  if (!acc && (rc = fill<double>(io.energy, (long long)io.batch * c->nseg, (long long)io.batch * c->nseg, 1, 0.0,
+  if (!acc && !chunk && (rc = fill<double>(io.energy, (long long)io.batch * c->nseg, (long long)io.batch * c->nseg, 1, 0.0,
                                  io.stream)))
     return rc;
   if (chunk) {
@@ -311,6 +311,14 @@ int femhot_autotune(femhot_ctx *h, int op, int dtype, int batch, const void *fie
                       grid_per_cu, ms);
   if (rc) return rc;
   return run_elem(c, io);  // leave valid outputs
+}
+
+int femhot_set_chunk_config(femhot_ctx *h, int op, int dtype, int batch, unsigned flags, int epc, int block,
+                            int grid_per_cu) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  FH_CHECK_ARG(op >= FEMHOT_OP_RESIDUAL && op <= FEMHOT_OP_ENERGY_GRAD, FEMHOT_ERR_ARG, "bad op");
+  FH_CHECK_ARG(dtype == FEMHOT_F32 || dtype == FEMHOT_F64, FEMHOT_ERR_ARG, "bad dtype");
+  return set_chunk_config(&h->c, op, dtype, batch, flags, epc, block, grid_per_cu);
 }
 
 }  // extern "C"

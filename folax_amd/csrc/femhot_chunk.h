@@ -18,6 +18,8 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
                    double *energy, unsigned flags, hipStream_t s, int *best_epc, int *best_block, int *best_grid,
                    float *best_ms);
 
+int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int epc, int block, int grid_per_cu);
+
 void forget_tuning(const void *ctx);
 
 }  // namespace femhot

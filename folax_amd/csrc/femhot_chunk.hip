@@ -7,11 +7,11 @@
 //      and gather the chunk's nodal solution / cotangent / field (node ids sorted) into LDS;
 //   2. one lane per element: element routine reading LDS, nodal results accumulated with LDS
 //      float atomics (ds_add), per-element results written straight to HBM;
-//   3. every chunk node is written once: plain store when the node belongs to this chunk only,
-//      hardware float atomic when it is shared with another chunk (those, and nodes in no element,
-//      are cleared first by k_zero_nodes).
-// HBM traffic per element is therefore the chunk-local node data once per chunk (instead of once
-// per element-node incidence) plus the element's own connectivity (uint16 local ids) and field.
+//   3. every chunk node is written once with a plain store: to the output when the node belongs
+//      to this chunk only, to the node's partial slot when it is shared with other chunks.
+// k_reduce_shared then sums each shared node's partials in fixed chunk order and k_reduce_energy
+// sums per-chunk energies per segment, so the path uses no global atomics (device-scope atomics
+// resolve beyond the per-XCD L2) and its results are bitwise reproducible run to run.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -24,10 +24,11 @@
 namespace femhot {
 
 struct ChunkArgs {
-  const int *elem_off, *node_off, *seg, *perm, *nodes;
+  const int *elem_off, *node_off, *perm, *nodes, *node_slot;
   const uint16_t *lconn;
   const void *xyz;  // chunk-relative coordinates in the arithmetic type
   int n_chunks, n_sblocks, sb, batch, cap;
+  long long n_slots;
   const void *field;
   long long fs;
   const void *u;
@@ -38,8 +39,9 @@ struct ChunkArgs {
   long long ous;
   void *od;
   long long ods;
-  double *energy;
-  int nseg;
+  void *part_u;          // (batch, n_slots, P) partials of shared nodes
+  void *part_d;          // (batch, n_slots) nodal field-gradient partials
+  double *chunk_energy;  // (batch, n_chunks)
   const uint8_t *dmask;
   unsigned flags;
   PhysParams pp;
@@ -99,6 +101,16 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       const R *Ub = reinterpret_cast<const R *>(A.u) + (long long)b * A.us;
       const R *Wb = reinterpret_cast<const R *>(A.w) + (long long)b * A.ws;
       const R *Fb = reinterpret_cast<const R *>(A.field) + (long long)b * A.fs;
+      // first element of this lane: its connectivity and field are fetched before the barrier
+      int p = e0 + tid;
+      int l_next[N] = {};
+      int e_next = 0;
+      R de_next = R(0);
+      if (p < e1) {
+        load_lconn<N>(A.lconn + (long long)p * N, l_next);
+        e_next = A.perm[p];
+        if (elemental) de_next = Fb[e_next];
+      }
       for (int i = tid; i < nl; i += BLOCK) {
         const int g = A.nodes[n0 + i] & 0x7fffffff;
 #pragma unroll
@@ -114,10 +126,17 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       }
       __syncthreads();
       double en = 0.0;
-      for (int p = e0 + tid; p < e1; p += BLOCK) {
-        const int e = A.perm[p];
+      for (; p < e1; p += BLOCK) {
         int l[N];
-        load_lconn<N>(A.lconn + (long long)p * N, l);
+#pragma unroll
+        for (int a = 0; a < N; ++a) l[a] = l_next[a];
+        const int e = e_next;
+        const R de = de_next;
+        if (p + BLOCK < e1) {  // prefetch the lane's next element
+          load_lconn<N>(A.lconn + (long long)(p + BLOCK) * N, l_next);
+          e_next = A.perm[p + BLOCK];
+          if (elemental) de_next = Fb[e_next];
+        }
         E el;
 #pragma unroll
         for (int i = 0; i < D; ++i) {
@@ -127,9 +146,9 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
         }
         el.elemental = elemental;
         if (elemental) {
-          el.de = Fb[e];
+          el.de = de;
 #pragma unroll
-          for (int a = 0; a < N; ++a) el.d[a] = el.de;
+          for (int a = 0; a < N; ++a) el.d[a] = de;
         } else {
           el.de = R(0);
 #pragma unroll
@@ -174,40 +193,41 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
           }
         }
       }
-      if (A.energy) {
+      if (A.chunk_energy) {
         en = wave_sum(en);
         if ((tid & 63) == 0) red[tid >> 6] = en;
       }
       __syncthreads();
-      if (A.energy && tid == 0) {
+      if (A.chunk_energy && tid == 0) {
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < BLOCK / 64; ++k) s += red[k];
-        unsafeAtomicAdd(&A.energy[(long long)b * A.nseg + A.seg[chunk]], s);
+        A.chunk_energy[(long long)b * A.n_chunks + chunk] = s;
       }
       R *OU = reinterpret_cast<R *>(A.ou) + (long long)b * A.ous;
       R *OD = reinterpret_cast<R *>(A.od) + (long long)b * A.ods;
+      R *PU = reinterpret_cast<R *>(A.part_u) + (long long)b * A.n_slots * P;
+      R *PD = reinterpret_cast<R *>(A.part_d) + (long long)b * A.n_slots;
       for (int i = tid; i < nl; i += BLOCK) {
-        const int raw = A.nodes[n0 + i];
-        const int g = raw & 0x7fffffff;
-        const bool shared = raw < 0;
+        const int g = A.nodes[n0 + i] & 0x7fffffff;
+        const int slot = A.node_slot[n0 + i];
         if (want_ou) {
 #pragma unroll
           for (int c = 0; c < P; ++c) {
-            const long long dof = (long long)g * P + c;
-            const bool masked = mask && A.dmask[dof];
             const R v = ACC[i * P + c];
-            if (shared) {
-              if (!masked) unsafeAtomicAdd(OU + dof, v);
-            } else if (!(masked && accumulate)) {
-              OU[dof] = masked ? R(0) : (accumulate ? OU[dof] + v : v);
+            if (slot >= 0) {
+              PU[(long long)slot * P + c] = v;
+            } else {
+              const long long dof = (long long)g * P + c;
+              const bool masked = mask && A.dmask[dof];
+              if (!(masked && accumulate)) OU[dof] = masked ? R(0) : (accumulate ? OU[dof] + v : v);
             }
           }
         }
         if (want_gd && !elemental) {
           const R v = GD[i];
-          if (shared)
-            unsafeAtomicAdd(OD + g, v);
+          if (slot >= 0)
+            PD[slot] = v;
           else
             OD[g] = accumulate ? OD[g] + v : v;
         }
@@ -217,8 +237,55 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   }
 }
 
-// Clears the outputs of shared nodes and of nodes in no element (the chunk kernel writes every
-// other node exactly once).
+// Shared nodes: sum the partials of every chunk containing the node, in chunk order.
+template <class R>
+__global__ __launch_bounds__(256) void k_reduce_shared(long long n_shared, int P, int batch, const int *sh_nodes,
+                                                       const int *red_off, const int *red_slots, long long n_slots,
+                                                       const R *part_u, const R *part_d, R *ou, long long ous, R *od,
+                                                       long long ods, const uint8_t *dmask, unsigned flags) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_shared * batch) return;
+  const long long b = i / n_shared;
+  const long long k = i - b * n_shared;
+  const int g = sh_nodes[k];
+  const int j0 = red_off[k], j1 = red_off[k + 1];
+  const bool mask = (flags & FEMHOT_MASK_DIRICHLET) != 0;
+  const bool accumulate = (flags & FEMHOT_ACCUMULATE) != 0;
+  if (ou) {
+    const R *pu = part_u + b * n_slots * P;
+    for (int c = 0; c < P; ++c) {
+      R s = R(0);
+      for (int j = j0; j < j1; ++j) s += pu[(long long)red_slots[j] * P + c];
+      const long long dof = (long long)g * P + c;
+      const bool masked = mask && dmask[dof];
+      R *o = ou + b * ous + dof;
+      if (!(masked && accumulate)) *o = masked ? R(0) : (accumulate ? *o + s : s);
+    }
+  }
+  if (od) {
+    const R *pd = part_d + b * n_slots;
+    R s = R(0);
+    for (int j = j0; j < j1; ++j) s += pd[red_slots[j]];
+    R *o = od + b * ods + g;
+    *o = accumulate ? *o + s : s;
+  }
+}
+
+// energy[b, seg] (+)= sum of the per-chunk energies of the segment's chunks, one wave per (b, seg).
+__global__ __launch_bounds__(64) void k_reduce_energy(int nseg, int n_chunks, const int *seg_chunk_off,
+                                                      const double *chunk_energy, double *energy, int accumulate) {
+  const int b = blockIdx.y, s = blockIdx.x;
+  double v = 0.0;
+  for (int c = seg_chunk_off[s] + (int)threadIdx.x; c < seg_chunk_off[s + 1]; c += 64)
+    v += chunk_energy[(long long)b * n_chunks + c];
+  v = wave_sum(v);
+  if (threadIdx.x == 0) {
+    double *o = energy + (long long)b * nseg + s;
+    *o = accumulate ? *o + v : v;
+  }
+}
+
+// Clears the outputs of nodes in no element (the chunk path writes every other node).
 template <class R>
 __global__ void k_zero_nodes(const int *nodes, long long n, int P, int batch, R *ou, long long ous, R *od,
                              long long ods) {
@@ -249,31 +316,29 @@ static int cu_count() {
   return cus;
 }
 
-// The dynamic-LDS limit is raised once per kernel (to the full 160 KiB) on its first launch, so
-// later launches (e.g. under graph capture) make no attribute calls.
-template <class K> static int raise_lds_limit(K k) {
-  static bool done = false;
-  if (!done) {
-    FH_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    done = true;
+// Largest dynamic LDS a workgroup may request on this device (less the kernel's static LDS).
+static size_t lds_limit() {
+  static size_t lim = 0;
+  if (!lim) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      v = 64 * 1024;
+    lim = (size_t)v - 256;
   }
-  return FEMHOT_OK;
+  return lim;
 }
 
 template <class R, int ET, int PH, int NGP, int MODE>
 static int launch_chunk_block(int block, dim3 grid, size_t shm, hipStream_t s, const ChunkArgs &A) {
-  int rc;
   if (block == 128) {
     auto k = k_chunk<R, ET, PH, NGP, MODE, 128>;
-    if ((rc = raise_lds_limit(k))) return rc;
     hipLaunchKernelGGL(k, grid, dim3(128), shm, s, A);
   } else if (block == 512) {
     auto k = k_chunk<R, ET, PH, NGP, MODE, 512>;
-    if ((rc = raise_lds_limit(k))) return rc;
     hipLaunchKernelGGL(k, grid, dim3(512), shm, s, A);
   } else {
     auto k = k_chunk<R, ET, PH, NGP, MODE, 256>;
-    if ((rc = raise_lds_limit(k))) return rc;
     hipLaunchKernelGGL(k, grid, dim3(256), shm, s, A);
   }
   FH_HIP(hipGetLastError());
@@ -303,13 +368,37 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   const size_t rs = dtype == FEMHOT_F32 ? sizeof(float) : sizeof(double);
   const int per_node = 3 + P + (mode == MODE_VJP ? P : 0) + (elemental ? 0 : 1) + P + (elemental ? 0 : 1);
   const size_t shm = (size_t)p.max_nodes * per_node * rs;
-  if (shm > 160 * 1024) {
+  if (shm > lds_limit()) {
     set_error("run_chunk: chunk too large for LDS; use fewer elements per chunk");
     return FEMHOT_ERR_UNSUPPORTED;
   }
   const bool acc = (flags & FEMHOT_ACCUMULATE) != 0;
   const bool want_gd_nodal = od && !elemental && mode != MODE_RES;
-  // clear shared / untouched nodes
+  // scratch: shared-node partials (batch x n_slots x P, + batch x n_slots for the nodal field
+  // gradient) and per-chunk energies (batch x n_chunks); grown here, never inside a capture
+  const size_t pu_bytes = ou ? ((size_t)batch * p.n_slots * P * rs + 255) / 256 * 256 : 0;
+  const size_t pd_bytes = want_gd_nodal ? ((size_t)batch * p.n_slots * rs + 255) / 256 * 256 : 0;
+  const size_t ce_bytes = energy ? ((size_t)batch * p.n_chunks * sizeof(double) + 255) / 256 * 256 : 0;
+  const size_t need = pu_bytes + pd_bytes + ce_bytes + 256;
+  if (c->chunk_ws.bytes < need) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FH_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+      set_error("run_chunk: scratch must be sized by an uncaptured call with the same batch before capture");
+      return FEMHOT_ERR_ARG;
+    }
+    FH_HIP(hipStreamSynchronize(s));
+    if (c->chunk_ws.ptr) FH_HIP(hipFree(c->chunk_ws.ptr));
+    c->chunk_ws.ptr = nullptr;
+    c->chunk_ws.bytes = 0;
+    FH_HIP(hipMalloc(&c->chunk_ws.ptr, need));
+    c->chunk_ws.bytes = need;
+  }
+  char *wsp = (char *)c->chunk_ws.ptr;
+  void *part_u = pu_bytes ? (void *)wsp : nullptr;
+  void *part_d = pd_bytes ? (void *)(wsp + pu_bytes) : nullptr;
+  double *chunk_energy = ce_bytes ? (double *)(wsp + pu_bytes + pd_bytes) : nullptr;
+  // clear nodes in no element
   if (!acc && p.n_zero && (ou || want_gd_nodal)) {
     const long long tot = p.n_zero * batch;
     const dim3 g((unsigned)((tot + 255) / 256));
@@ -324,9 +413,9 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   ChunkArgs A{};
   A.elem_off = p.chunk_elem_off;
   A.node_off = p.chunk_node_off;
-  A.seg = p.chunk_seg;
   A.perm = p.perm;
   A.nodes = p.chunk_nodes;
+  A.node_slot = p.node_slot;
   A.lconn = p.lconn;
   A.xyz = dtype == FEMHOT_F32 ? (const void *)p.chunk_xyz32 : (const void *)p.chunk_xyz64;
   A.n_chunks = p.n_chunks;
@@ -334,6 +423,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   A.n_sblocks = (batch + A.sb - 1) / A.sb;
   A.batch = batch;
   A.cap = p.max_nodes;
+  A.n_slots = p.n_slots;
   A.field = field;
   A.fs = batch > 1 ? fs : 0;
   A.u = u;
@@ -344,8 +434,9 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   A.ous = batch > 1 ? ous : c->Ndof;
   A.od = mode == MODE_RES ? nullptr : od;
   A.ods = batch > 1 ? ods : (elemental ? c->Ne : c->Nn);
-  A.energy = energy;
-  A.nseg = c->nseg;
+  A.part_u = part_u;
+  A.part_d = part_d;
+  A.chunk_energy = chunk_energy;
   A.dmask = c->d_dmask;
   A.flags = flags;
   A.pp = c->pp;
@@ -360,7 +451,27 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
     default: rc = launch_chunk_et<TET4>(dtype, c->ph, c->ngp, mode, cfg.block, dim3((unsigned)grid), shm, s, A); break;
   }
   if (rc) return rc;
-  return timing_mark(c, s);
+  // deterministic reductions of shared nodes and per-chunk energies
+  if (p.n_shared && (ou || want_gd_nodal)) {
+    const long long tot = p.n_shared * batch;
+    const dim3 g((unsigned)((tot + 255) / 256));
+    const long long os = batch > 1 ? ous : c->Ndof, dstr = batch > 1 ? ods : c->Nn;
+    if (dtype == FEMHOT_F32)
+      hipLaunchKernelGGL(k_reduce_shared<float>, g, dim3(256), 0, s, p.n_shared, P, batch, p.sh_nodes, p.red_off,
+                         p.red_slots, p.n_slots, (const float *)part_u, (const float *)part_d, (float *)ou, os,
+                         want_gd_nodal ? (float *)od : nullptr, dstr, c->d_dmask, flags);
+    else
+      hipLaunchKernelGGL(k_reduce_shared<double>, g, dim3(256), 0, s, p.n_shared, P, batch, p.sh_nodes, p.red_off,
+                         p.red_slots, p.n_slots, (const double *)part_u, (const double *)part_d, (double *)ou, os,
+                         want_gd_nodal ? (double *)od : nullptr, dstr, c->d_dmask, flags);
+    FH_HIP(hipGetLastError());
+  }
+  if (energy) {
+    hipLaunchKernelGGL(k_reduce_energy, dim3(c->nseg, batch), dim3(64), 0, s, c->nseg, p.n_chunks, p.seg_chunk_off,
+                       chunk_energy, energy, acc ? 1 : 0);
+    FH_HIP(hipGetLastError());
+  }
+  return timing_mark(c, s);  // the timed region covers the chunk kernel and its reductions
 }
 
 static ChunkConfig default_config(Ctx *c, int dtype) {
@@ -442,6 +553,22 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
   if (best_grid) *best_grid = best.grid_per_cu;
   if (best_ms) *best_ms = best_t;
   return FEMHOT_OK;
+}
+
+int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int epc, int block, int grid_per_cu) {
+  if (epc < 1 || epc > 8192 || (block != 128 && block != 256 && block != 512) || grid_per_cu < 0) {
+    set_error("femhot_set_chunk_config: epc in [1,8192], block in {128,256,512}, grid_per_cu >= 0");
+    return FEMHOT_ERR_ARG;
+  }
+  ChunkConfig cfg;
+  cfg.epc = epc;
+  cfg.block = block;
+  cfg.grid_per_cu = grid_per_cu;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[{c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)}] = cfg;
+  }
+  return ensure_chunks(c, epc);
 }
 
 void forget_tuning(const void *ctx) {

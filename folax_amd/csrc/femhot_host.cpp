@@ -163,6 +163,11 @@ static void free_chunks(ChunkPlan &p) {
   (void)hipFree(p.chunk_xyz32);
   (void)hipFree(p.chunk_xyz64);
   (void)hipFree(p.zero_nodes);
+  (void)hipFree(p.node_slot);
+  (void)hipFree(p.sh_nodes);
+  (void)hipFree(p.red_off);
+  (void)hipFree(p.red_slots);
+  (void)hipFree(p.seg_chunk_off);
   p = ChunkPlan{};
 }
 
@@ -258,26 +263,52 @@ int ensure_chunks(Ctx *c, int epc) {
   }
   std::vector<int> touch(c->Nn, 0);
   for (int g : nodes) touch[g]++;
-  std::vector<int> zero;
-  long long n_shared = 0;
+  std::vector<int> zero, sh_nodes;
+  std::vector<int> sh_index(c->Nn, -1);
   for (long long g = 0; g < c->Nn; ++g) {
-    if (touch[g] != 1) zero.push_back((int)g);
-    n_shared += touch[g] > 1;
+    if (touch[g] == 0) zero.push_back((int)g);
+    if (touch[g] > 1) {
+      sh_index[g] = (int)sh_nodes.size();
+      sh_nodes.push_back((int)g);
+    }
   }
-  for (int &g : nodes)
-    if (touch[g] > 1) g = (int)((unsigned)g | 0x80000000u);
+  // partial slots: chunk by chunk, in local-node order; per shared node, its slots in chunk order
+  std::vector<int> node_slot(nodes.size(), -1);
+  std::vector<int> red_off(sh_nodes.size() + 1, 0);
+  for (int g : sh_nodes) red_off[sh_index[g] + 1] = touch[g];
+  for (size_t k = 0; k < sh_nodes.size(); ++k) red_off[k + 1] += red_off[k];
+  std::vector<int> red_slots(red_off.back());
+  {
+    std::vector<int> fillpos(red_off.begin(), red_off.end() - 1);
+    int slot = 0;
+    for (size_t k = 0; k < nodes.size(); ++k) {
+      const int g = nodes[k];
+      if (touch[g] > 1) {
+        node_slot[k] = slot;
+        red_slots[fillpos[sh_index[g]]++] = slot;
+        ++slot;
+      }
+    }
+  }
+  std::vector<int> seg_chunk_off(c->nseg + 1, 0);
+  for (int s : seg) seg_chunk_off[s + 1]++;
+  for (int s = 0; s < c->nseg; ++s) seg_chunk_off[s + 1] += seg_chunk_off[s];
   ChunkPlan &p = c->chunk;
   p.epc = epc;
   p.n_chunks = (int)seg.size();
   p.max_nodes = max_nodes;
   p.max_elems = max_elems;
-  p.n_shared = n_shared;
+  p.n_shared = (long long)sh_nodes.size();
   p.n_zero = (long long)zero.size();
+  p.n_slots = (long long)red_slots.size();
   int rc;
   if ((rc = upload(&p.chunk_elem_off, elem_off)) || (rc = upload(&p.chunk_node_off, node_off)) ||
       (rc = upload(&p.chunk_seg, seg)) || (rc = upload(&p.perm, perm)) || (rc = upload(&p.lconn, lconn)) ||
       (rc = upload(&p.chunk_nodes, nodes)) || (rc = upload(&p.chunk_xyz32, xyz32)) ||
-      (rc = upload(&p.chunk_xyz64, xyz64)) || (rc = upload(&p.zero_nodes, zero))) {
+      (rc = upload(&p.chunk_xyz64, xyz64)) || (rc = upload(&p.zero_nodes, zero)) ||
+      (rc = upload(&p.node_slot, node_slot)) || (rc = upload(&p.sh_nodes, sh_nodes)) ||
+      (rc = upload(&p.red_off, red_off)) || (rc = upload(&p.red_slots, red_slots)) ||
+      (rc = upload(&p.seg_chunk_off, seg_chunk_off))) {
     free_chunks(p);
     return rc;
   }
@@ -474,6 +505,7 @@ void femhot_ctx_destroy(femhot_ctx *h) {
   (void)hipFree(c.d_slot);
   (void)hipFree(c.d_colour_elems);
   free_chunks(c.chunk);
+  (void)hipFree(c.chunk_ws.ptr);
   (void)hipFree(c.d_ws);
   for (hipEvent_t e : c.tev_pool) (void)hipEventDestroy(e);
   if (c.h_pinned) (void)hipHostFree(c.h_pinned);

@@ -49,8 +49,21 @@ struct ChunkPlan {
   int *chunk_nodes = nullptr;     // global node id | 0x80000000 when shared
   float *chunk_xyz32 = nullptr;   // per chunk node: coordinates relative to the chunk's first node
   double *chunk_xyz64 = nullptr;
-  long long n_zero = 0;           // nodes whose outputs must be cleared before a chunk launch
-  int *zero_nodes = nullptr;      // shared nodes + nodes in no element
+  long long n_zero = 0;           // nodes in no element (their outputs are cleared separately)
+  int *zero_nodes = nullptr;
+  // deterministic reduction of shared nodes: each (chunk, shared node) pair owns a partial slot
+  long long n_slots = 0;
+  int *node_slot = nullptr;       // per chunk node: partial slot, -1 when the node is exclusive
+  int *sh_nodes = nullptr;        // (n_shared) shared node ids, ascending
+  int *red_off = nullptr;         // (n_shared+1) offsets into red_slots
+  int *red_slots = nullptr;       // (n_slots) slots of each shared node, in chunk order
+  int *seg_chunk_off = nullptr;   // (nseg+1) chunks of each segment (chunks are segment-sorted)
+};
+
+// Device scratch of the chunk path (partials and per-chunk energies), grown on demand.
+struct ChunkWork {
+  size_t bytes = 0;
+  void *ptr = nullptr;
 };
 
 struct Ctx {
@@ -85,6 +98,7 @@ struct Ctx {
   std::vector<int> h_colour_off;  // (n_colours+1)
   int *d_colour_elems = nullptr;  // elements sorted by colour (stable)
   ChunkPlan chunk;                // built on demand
+  ChunkWork chunk_ws;
   // kernel timing hook: events recorded around the dominant kernel of each call while enabled
   bool timing = false;
   std::vector<hipEvent_t> tev_pool;

@@ -155,6 +155,12 @@ int femhot_autotune(femhot_ctx *ctx, int op, int dtype, int batch, const void *f
                     long long out_u_stride, void *out_field, long long out_field_stride, double *energy,
                     unsigned flags, int *epc, int *block, int *grid_per_cu, float *ms, void *stream);
 
+/* Sets the chunk configuration femhot_autotune would record (e.g. one saved from an earlier
+ * process) for (ctx, op, dtype, batch, field location) and builds its chunk plan now, so later
+ * calls (and graph captures) allocate nothing.  No counterpart in the reference. */
+int femhot_set_chunk_config(femhot_ctx *ctx, int op, int dtype, int batch, unsigned flags, int epc, int block,
+                            int grid_per_cu);
+
 /* Measurement hook (no counterpart in the reference): enable=1 starts recording HIP events on the
  * call's stream immediately before and after the dominant element kernel(s) of every residual /
  * VJP / potential-gradient call on this context; enable=0 waits for them, returns the summed

@@ -83,9 +83,10 @@ def test_chunk_batched_accumulate_and_epc(dt):
     from folax_amd.context import _ptr, _stream
     acc_u = dev(np.ones((B, Nn)), dt)
     acc_d = dev(np.full((B, Nn), 2.0), dt)
-    L.check(ctx._lib.femhot_residual_vjp(ctx._h, L.F32 if dt == "f32" else L.F64, B, _ptr(dev(F, dt)), Nn,
-                                         _ptr(dev(U, dt)), Nn, _ptr(dev(W, dt)), Nn, _ptr(acc_u), Nn, _ptr(acc_d), Nn,
-                                         CHUNK | L.ACCUMULATE, _stream()))
+    Ft, Ut, Wt = dev(F, dt), dev(U, dt), dev(W, dt)  # keep the inputs alive across the raw call
+    L.check(ctx._lib.femhot_residual_vjp(ctx._h, L.F32 if dt == "f32" else L.F64, B, _ptr(Ft), Nn, _ptr(Ut), Nn,
+                                         _ptr(Wt), Nn, _ptr(acc_u), Nn, _ptr(acc_d), Nn, CHUNK | L.ACCUMULATE,
+                                         _stream()))
     torch.cuda.synchronize()
     for b in range(B):
         g1, g2 = fe.global_residual_vjp(ph, case["coords"], case["conn"], F[b], U[b], W[b])
