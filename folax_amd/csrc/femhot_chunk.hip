@@ -19,223 +19,9 @@
 #include <vector>
 
 #include "femhot_chunk.h"
-#include "femhot_launch.h"
+#include "femhot_chunk_kernels.h"
 
 namespace femhot {
-
-struct ChunkArgs {
-  const int *elem_off, *node_off, *perm, *nodes, *node_slot;
-  const uint16_t *lconn;
-  const void *xyz;  // chunk-relative coordinates in the arithmetic type
-  int n_chunks, n_sblocks, sb, batch, cap;
-  long long n_slots;
-  const void *field;
-  long long fs;
-  const void *u;
-  long long us;
-  const void *w;
-  long long ws;
-  void *ou;
-  long long ous;
-  void *od;
-  long long ods;
-  void *part_u;          // (batch, n_slots, P) partials of shared nodes
-  void *part_d;          // (batch, n_slots) nodal field-gradient partials
-  double *chunk_energy;  // (batch, n_chunks)
-  const uint8_t *dmask;
-  unsigned flags;
-  PhysParams pp;
-};
-
-template <int N> __device__ __forceinline__ void load_lconn(const uint16_t *p, int (&l)[N]) {
-  if constexpr (N == 4) {
-    const uint2 v = *reinterpret_cast<const uint2 *>(p);
-    l[0] = v.x & 0xffff;
-    l[1] = v.x >> 16;
-    l[2] = v.y & 0xffff;
-    l[3] = v.y >> 16;
-  } else {
-    const uint4 v = *reinterpret_cast<const uint4 *>(p);
-    l[0] = v.x & 0xffff;
-    l[1] = v.x >> 16;
-    l[2] = v.y & 0xffff;
-    l[3] = v.y >> 16;
-    l[4] = v.z & 0xffff;
-    l[5] = v.z >> 16;
-    l[6] = v.w & 0xffff;
-    l[7] = v.w >> 16;
-  }
-}
-
-template <class R, int ET, int PH, int NGP, int MODE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
-  using E = Elem<R, ET, PH>;
-  constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ double red[BLOCK / 64];
-  const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
-  const bool mask = (A.flags & FEMHOT_MASK_DIRICHLET) != 0;
-  const bool accumulate = (A.flags & FEMHOT_ACCUMULATE) != 0;
-  const bool picard = (A.flags & FEMHOT_PICARD) != 0;
-  const int cap = A.cap;
-  // LDS layout (all arrays sized for `cap` nodes)
-  R *X = reinterpret_cast<R *>(smem_raw);
-  R *U = X + 3 * cap;
-  R *W = U + P * cap;                              // cotangent (VJP only)
-  R *FD = W + (MODE == MODE_VJP ? P * cap : 0);    // nodal field
-  R *ACC = FD + (elemental ? 0 : cap);             // nodal output accumulators
-  R *GD = ACC + P * cap;                           // nodal field-gradient accumulators
-  const bool want_gd = (MODE != MODE_RES) && A.od != nullptr;
-  const bool want_ou = A.ou != nullptr;
-  const int tid = threadIdx.x;
-  const int n_items = A.n_chunks * A.n_sblocks;
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
-    const int chunk = item / A.n_sblocks;
-    const int sbi = item - chunk * A.n_sblocks;
-    const int e0 = A.elem_off[chunk], e1 = A.elem_off[chunk + 1];
-    const int n0 = A.node_off[chunk], nl = A.node_off[chunk + 1] - n0;
-    const R *xyz = reinterpret_cast<const R *>(A.xyz) + 3LL * n0;
-    for (int i = tid; i < 3 * nl; i += BLOCK) X[i] = xyz[i];
-    const int b0 = sbi * A.sb, b1 = min(A.batch, b0 + A.sb);
-    for (int b = b0; b < b1; ++b) {
-      const R *Ub = reinterpret_cast<const R *>(A.u) + (long long)b * A.us;
-      const R *Wb = reinterpret_cast<const R *>(A.w) + (long long)b * A.ws;
-      const R *Fb = reinterpret_cast<const R *>(A.field) + (long long)b * A.fs;
-      // first element of this lane: its connectivity and field are fetched before the barrier
-      int p = e0 + tid;
-      int l_next[N] = {};
-      int e_next = 0;
-      R de_next = R(0);
-      if (p < e1) {
-        load_lconn<N>(A.lconn + (long long)p * N, l_next);
-        e_next = A.perm[p];
-        if (elemental) de_next = Fb[e_next];
-      }
-      for (int i = tid; i < nl; i += BLOCK) {
-        const int g = A.nodes[n0 + i] & 0x7fffffff;
-#pragma unroll
-        for (int c = 0; c < P; ++c) {
-          U[i * P + c] = Ub[(long long)g * P + c];
-          if constexpr (MODE == MODE_VJP) W[i * P + c] = Wb[(long long)g * P + c];
-          ACC[i * P + c] = R(0);
-        }
-        if (!elemental) {
-          FD[i] = Fb[g];
-          GD[i] = R(0);
-        }
-      }
-      __syncthreads();
-      double en = 0.0;
-      for (; p < e1; p += BLOCK) {
-        int l[N];
-#pragma unroll
-        for (int a = 0; a < N; ++a) l[a] = l_next[a];
-        const int e = e_next;
-        const R de = de_next;
-        if (p + BLOCK < e1) {  // prefetch the lane's next element
-          load_lconn<N>(A.lconn + (long long)(p + BLOCK) * N, l_next);
-          e_next = A.perm[p + BLOCK];
-          if (elemental) de_next = Fb[e_next];
-        }
-        E el;
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-          const R x0 = X[l[0] * 3 + i];
-#pragma unroll
-          for (int a = 0; a < N; ++a) el.x[a][i] = X[l[a] * 3 + i] - x0;
-        }
-        el.elemental = elemental;
-        if (elemental) {
-          el.de = de;
-#pragma unroll
-          for (int a = 0; a < N; ++a) el.d[a] = de;
-        } else {
-          el.de = R(0);
-#pragma unroll
-          for (int a = 0; a < N; ++a) el.d[a] = FD[l[a]];
-        }
-#pragma unroll
-        for (int a = 0; a < N; ++a)
-#pragma unroll
-          for (int c = 0; c < P; ++c) el.u[a * P + c] = U[l[a] * P + c];
-        R ro[NL], gd[N];
-        if constexpr (MODE == MODE_RES) {
-          R energy;
-          elem_residual<R, ET, PH, NGP>(el, A.pp, ro, energy);
-          en += (double)energy;
-        } else if constexpr (MODE == MODE_VJP) {
-          R w[NL];
-#pragma unroll
-          for (int a = 0; a < N; ++a)
-#pragma unroll
-            for (int c = 0; c < P; ++c) w[a * P + c] = W[l[a] * P + c];
-          elem_vjp<R, ET, PH, NGP>(el, w, A.pp, picard, ro, gd);
-        } else {
-          R pi;
-          elem_egrad<R, ET, PH, NGP>(el, A.pp, ro, gd, pi);
-          en += (double)pi;
-        }
-        if (want_ou) {
-#pragma unroll
-          for (int a = 0; a < N; ++a)
-#pragma unroll
-            for (int c = 0; c < P; ++c) atomicAdd(&ACC[l[a] * P + c], ro[a * P + c]);
-        }
-        if constexpr (MODE != MODE_RES) {
-          if (want_gd) {
-            if (elemental) {
-              R *OD = reinterpret_cast<R *>(A.od) + (long long)b * A.ods;
-              OD[e] = accumulate ? OD[e] + gd[0] : gd[0];
-            } else {
-#pragma unroll
-              for (int a = 0; a < N; ++a) atomicAdd(&GD[l[a]], gd[a]);
-            }
-          }
-        }
-      }
-      if (A.chunk_energy) {
-        en = wave_sum(en);
-        if ((tid & 63) == 0) red[tid >> 6] = en;
-      }
-      __syncthreads();
-      if (A.chunk_energy && tid == 0) {
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < BLOCK / 64; ++k) s += red[k];
-        A.chunk_energy[(long long)b * A.n_chunks + chunk] = s;
-      }
-      R *OU = reinterpret_cast<R *>(A.ou) + (long long)b * A.ous;
-      R *OD = reinterpret_cast<R *>(A.od) + (long long)b * A.ods;
-      R *PU = reinterpret_cast<R *>(A.part_u) + (long long)b * A.n_slots * P;
-      R *PD = reinterpret_cast<R *>(A.part_d) + (long long)b * A.n_slots;
-      for (int i = tid; i < nl; i += BLOCK) {
-        const int g = A.nodes[n0 + i] & 0x7fffffff;
-        const int slot = A.node_slot[n0 + i];
-        if (want_ou) {
-#pragma unroll
-          for (int c = 0; c < P; ++c) {
-            const R v = ACC[i * P + c];
-            if (slot >= 0) {
-              PU[(long long)slot * P + c] = v;
-            } else {
-              const long long dof = (long long)g * P + c;
-              const bool masked = mask && A.dmask[dof];
-              if (!(masked && accumulate)) OU[dof] = masked ? R(0) : (accumulate ? OU[dof] + v : v);
-            }
-          }
-        }
-        if (want_gd && !elemental) {
-          const R v = GD[i];
-          if (slot >= 0)
-            PD[slot] = v;
-          else
-            OD[g] = accumulate ? OD[g] + v : v;
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
 
 // Shared nodes: sum the partials of every chunk containing the node, in chunk order.
 template <class R>
@@ -329,34 +115,6 @@ static size_t lds_limit() {
   return lim;
 }
 
-template <class R, int ET, int PH, int NGP, int MODE>
-static int launch_chunk_block(int block, dim3 grid, size_t shm, hipStream_t s, const ChunkArgs &A) {
-  if (block == 128) {
-    auto k = k_chunk<R, ET, PH, NGP, MODE, 128>;
-    hipLaunchKernelGGL(k, grid, dim3(128), shm, s, A);
-  } else if (block == 512) {
-    auto k = k_chunk<R, ET, PH, NGP, MODE, 512>;
-    hipLaunchKernelGGL(k, grid, dim3(512), shm, s, A);
-  } else {
-    auto k = k_chunk<R, ET, PH, NGP, MODE, 256>;
-    hipLaunchKernelGGL(k, grid, dim3(256), shm, s, A);
-  }
-  FH_HIP(hipGetLastError());
-  return FEMHOT_OK;
-}
-
-template <int ET>
-static int launch_chunk_et(int dtype, int ph, int ngp, int mode, int block, dim3 grid, size_t shm, hipStream_t s,
-                           const ChunkArgs &A) {
-  return dispatch_r_ph_ngp(dtype, ph, ngp, [&](auto rp, auto phc, auto ngpc) -> int {
-    using R = std::remove_pointer_t<decltype(rp)>;
-    constexpr int PH = decltype(phc)::value, NGP = decltype(ngpc)::value;
-    if (mode == MODE_RES) return launch_chunk_block<R, ET, PH, NGP, MODE_RES>(block, grid, shm, s, A);
-    if (mode == MODE_VJP) return launch_chunk_block<R, ET, PH, NGP, MODE_VJP>(block, grid, shm, s, A);
-    return launch_chunk_block<R, ET, PH, NGP, MODE_EGRAD>(block, grid, shm, s, A);
-  });
-}
-
 static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *field, long long fs, const void *u,
                          long long us, const void *w, long long ws, void *ou, long long ous, void *od, long long ods,
                          double *energy, unsigned flags, hipStream_t s, const ChunkConfig &cfg) {
@@ -366,8 +124,11 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   const bool elemental = (flags & FEMHOT_ELEMENT_FIELD) != 0;
   const int P = c->ndpn;
   const size_t rs = dtype == FEMHOT_F32 ? sizeof(float) : sizeof(double);
-  const int per_node = 3 + P + (mode == MODE_VJP ? P : 0) + (elemental ? 0 : 1) + P + (elemental ? 0 : 1);
-  const size_t shm = (size_t)p.max_nodes * per_node * rs;
+  const int PP = P == 3 ? 4 : P;  // padded per-node record (see k_chunk)
+  const bool nodal_gd = od && !elemental && mode != MODE_RES;
+  const size_t cap4 = ((size_t)p.max_nodes + 3) & ~(size_t)3;
+  const size_t shm = cap4 * (4 + PP + (mode == MODE_VJP ? PP : 0) + (elemental ? 0 : 1)) * rs +
+                     (size_t)p.max_elems * c->n * (PP + (nodal_gd ? 1 : 0)) * rs + (size_t)p.max_elems * c->n * 2 + 16;
   if (shm > lds_limit()) {
     set_error("run_chunk: chunk too large for LDS; use fewer elements per chunk");
     return FEMHOT_ERR_UNSUPPORTED;
@@ -417,12 +178,15 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   A.nodes = p.chunk_nodes;
   A.node_slot = p.node_slot;
   A.lconn = p.lconn;
+  A.inc = p.inc;
+  A.inc_start = p.inc_start;
   A.xyz = dtype == FEMHOT_F32 ? (const void *)p.chunk_xyz32 : (const void *)p.chunk_xyz64;
   A.n_chunks = p.n_chunks;
   A.sb = cfg.sb > 0 ? std::min(cfg.sb, batch) : batch;
   A.n_sblocks = (batch + A.sb - 1) / A.sb;
   A.batch = batch;
   A.cap = p.max_nodes;
+  A.emax = p.max_elems;
   A.n_slots = p.n_slots;
   A.field = field;
   A.fs = batch > 1 ? fs : 0;
@@ -474,14 +238,17 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   return timing_mark(c, s);  // the timed region covers the chunk kernel and its reductions
 }
 
+// Default chunk size: the largest power of two <= 1024 whose per-corner LDS buffer stays within
+// 48 KB, so several workgroups share a CU.
 static ChunkConfig default_config(Ctx *c, int dtype) {
   ChunkConfig cfg;
-  const bool big = c->et == HEX8 || (dtype == FEMHOT_F64 && c->ph == NEOHOOKE);
-  cfg.epc = big ? 256 : 1024;
+  const int PP = c->ndpn == 3 ? 4 : c->ndpn;
+  const size_t per_elem = (size_t)c->n * (PP + 1) * (dtype == FEMHOT_F64 ? 8 : 4);
+  cfg.epc = 1024;
+  while (cfg.epc > 64 && cfg.epc * per_elem > 48 * 1024) cfg.epc /= 2;
   cfg.block = 256;
   return cfg;
 }
-
 int run_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, long long fs, const void *u, long long us,
               const void *w, long long ws, void *ou, long long ous, void *od, long long ods, double *energy,
               unsigned flags, hipStream_t stream, int epc, int block) {
@@ -491,16 +258,30 @@ int run_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, long lo
     auto it = g_tuned.find({c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)});
     if (it != g_tuned.end()) cfg = it->second;
   }
-  if (epc > 0) cfg.epc = epc;
   if (block > 0) cfg.block = block;
-  return run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, stream, cfg);
+  if (epc > 0) {
+    cfg.epc = epc;
+    return run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, stream, cfg);
+  }
+  // a chunk that does not fit LDS (or the 16-bit incidence range) is retried with half the size;
+  // the size that worked is remembered for this call shape
+  for (bool halved = false;;) {
+    int rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, stream, cfg);
+    if (rc == FEMHOT_OK && halved) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tuned[{c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)}] = cfg;
+    }
+    if (rc != FEMHOT_ERR_UNSUPPORTED || cfg.epc <= 32) return rc;
+    cfg.epc /= 2;
+    halved = true;
+  }
 }
 
 int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, long long fs, const void *u,
                    long long us, const void *w, long long ws, void *ou, long long ous, void *od, long long ods,
                    double *energy, unsigned flags, hipStream_t s, int *best_epc, int *best_block, int *best_grid,
                    float *best_ms) {
-  static const int epcs[] = {256, 512, 1024, 2048, 4096};
+  static const int epcs[] = {128, 256, 512, 1024, 2048};
   static const int blocks[] = {128, 256, 512};
   static const int grids[] = {0, 4, 8};
   hipEvent_t e0, e1;
@@ -509,7 +290,7 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
   ChunkConfig best;
   float best_t = 1e30f;
   for (int epc : epcs) {
-    if ((long long)epc > c->Ne * 2 && epc != 256) continue;
+    if ((long long)epc > c->Ne * 2 && epc != 128) continue;
     for (int blk : blocks)
       for (int gp : grids) {
         ChunkConfig cfg;

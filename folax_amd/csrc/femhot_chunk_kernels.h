@@ -1,0 +1,363 @@
+// Chunk-path kernel template and its launcher (instantiated per element type in
+// femhot_chunk_<type>.hip so the build compiles them in parallel).
+#pragma once
+#include "femhot_launch.h"
+
+namespace femhot {
+
+struct ChunkArgs {
+  const int *elem_off, *node_off, *perm, *nodes, *node_slot;
+  const uint16_t *lconn, *inc, *inc_start;
+  const void *xyz;  // chunk-relative coordinates in the arithmetic type
+  int n_chunks, n_sblocks, sb, batch, cap, emax;
+  long long n_slots;
+  const void *field;
+  long long fs;
+  const void *u;
+  long long us;
+  const void *w;
+  long long ws;
+  void *ou;
+  long long ous;
+  void *od;
+  long long ods;
+  void *part_u;          // (batch, n_slots, P) partials of shared nodes
+  void *part_d;          // (batch, n_slots) nodal field-gradient partials
+  double *chunk_energy;  // (batch, n_chunks)
+  const uint8_t *dmask;
+  unsigned flags;
+  PhysParams pp;
+};
+
+template <int N> __device__ __forceinline__ void load_lconn(const uint16_t *p, int (&l)[N]) {
+  if constexpr (N == 4) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(p);
+    l[0] = v.x & 0xffff;
+    l[1] = v.x >> 16;
+    l[2] = v.y & 0xffff;
+    l[3] = v.y >> 16;
+  } else {
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    l[0] = v.x & 0xffff;
+    l[1] = v.x >> 16;
+    l[2] = v.y & 0xffff;
+    l[3] = v.y >> 16;
+    l[4] = v.z & 0xffff;
+    l[5] = v.z >> 16;
+    l[6] = v.w & 0xffff;
+    l[7] = v.w >> 16;
+  }
+}
+
+// Per-node LDS records are padded to 4 values when P == 3 so they are read / written with one
+// 16-byte ds access; PP is the padded stride.
+template <int P> struct Pad { static constexpr int PP = (P == 3) ? 4 : P; };
+
+// LDS layout of k_chunk (in units of R unless noted; cap4 = max chunk nodes rounded up to 4,
+// emax = max chunk elements):
+//   X   [cap4 * 4]      chunk-relative coordinates (x, y, z, 0)
+//   U   [cap4 * PP]     nodal solution
+//   W   [cap4 * PP]     cotangent (VJP only)
+//   FD  [cap4]          nodal field (nodal fields only)
+//   ER  [emax * N * PP] per element corner: residual / g_u contribution
+//   ERD [emax * N]      per element corner: nodal field-gradient contribution (VJP / EGRAD, nodal)
+//   INC [emax * N] (uint16) the chunk's incidence list
+template <class R, int ET, int PH, int NGP, int MODE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
+  using E = Elem<R, ET, PH>;
+  constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL, PP = Pad<P>::PP;
+  constexpr int KN = 4;                                  // nodes per lane per round
+  constexpr int KE = (ET != HEX8 && NGP <= 2) ? 4 : 1;  // elements per lane per round
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double red[BLOCK / 64];
+  const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
+  const bool mask = (A.flags & FEMHOT_MASK_DIRICHLET) != 0;
+  const bool accumulate = (A.flags & FEMHOT_ACCUMULATE) != 0;
+  const bool picard = (A.flags & FEMHOT_PICARD) != 0;
+  const bool want_gd = (MODE != MODE_RES) && A.od != nullptr;
+  const bool want_ou = A.ou != nullptr;
+  const bool nodal_gd = want_gd && !elemental;
+  const int cap4 = (A.cap + 3) & ~3, emax = A.emax;
+  R *X = reinterpret_cast<R *>(smem_raw);
+  R *U = X + 4 * cap4;
+  R *W = U + PP * cap4;
+  R *FD = W + (MODE == MODE_VJP ? PP * cap4 : 0);
+  R *ER = FD + (elemental ? 0 : cap4);
+  R *ERD = ER + (long)emax * N * PP;
+  uint16_t *INC = reinterpret_cast<uint16_t *>(ERD + (nodal_gd ? (long)emax * N : 0));
+  const int tid = threadIdx.x;
+  const int n_items = A.n_chunks * A.n_sblocks;
+  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int chunk = item / A.n_sblocks;
+    const int sbi = item - chunk * A.n_sblocks;
+    const int e0 = A.elem_off[chunk], e1 = A.elem_off[chunk + 1];
+    const int ne = e1 - e0;
+    const int n0 = A.node_off[chunk], nl = A.node_off[chunk + 1] - n0;
+    {  // coordinates (one 16-/32-byte record per node) and the incidence list
+      const R *xyz = reinterpret_cast<const R *>(A.xyz) + 4LL * n0;
+      for (int i = tid; i < nl; i += BLOCK) {
+        if constexpr (sizeof(R) == 4) {
+          reinterpret_cast<float4 *>(X)[i] = reinterpret_cast<const float4 *>(xyz)[i];
+        } else {
+          reinterpret_cast<double2 *>(X)[2 * i] = reinterpret_cast<const double2 *>(xyz)[2 * i];
+          reinterpret_cast<double2 *>(X)[2 * i + 1] = reinterpret_cast<const double2 *>(xyz)[2 * i + 1];
+        }
+      }
+      if (want_ou || nodal_gd) {
+        const uint16_t *inc = A.inc + (long long)N * e0;
+        for (int j = tid; j < N * ne; j += BLOCK) INC[j] = inc[j];
+      }
+    }
+    const int b0 = sbi * A.sb, b1 = min(A.batch, b0 + A.sb);
+    for (int b = b0; b < b1; ++b) {
+      const R *Ub = reinterpret_cast<const R *>(A.u) + (long long)b * A.us;
+      const R *Wb = reinterpret_cast<const R *>(A.w) + (long long)b * A.ws;
+      const R *Fb = reinterpret_cast<const R *>(A.field) + (long long)b * A.fs;
+      // stage nodal solution / cotangent / field; indices are clamped (not branched on) so every
+      // load of a round is issued back to back
+      for (int base = 0; base < nl; base += KN * BLOCK) {
+        int g[KN];
+#pragma unroll
+        for (int k = 0; k < KN; ++k) g[k] = A.nodes[n0 + min(base + k * BLOCK + tid, nl - 1)] & 0x7fffffff;
+        R uv[KN][P], wv[KN][P], fv[KN];
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+#pragma unroll
+          for (int c = 0; c < P; ++c) {
+            uv[k][c] = Ub[(long long)g[k] * P + c];
+            if constexpr (MODE == MODE_VJP) wv[k][c] = Wb[(long long)g[k] * P + c];
+          }
+          fv[k] = elemental ? R(0) : Fb[g[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+          const int i = base + k * BLOCK + tid;
+          if (i < nl) {
+#pragma unroll
+            for (int c = 0; c < PP; ++c) {
+              U[i * PP + c] = c < P ? uv[k][c < P ? c : 0] : R(0);
+              if constexpr (MODE == MODE_VJP) W[i * PP + c] = c < P ? wv[k][c < P ? c : 0] : R(0);
+            }
+            if (!elemental) FD[i] = fv[k];
+          }
+        }
+      }
+      __syncthreads();
+      double en = 0.0;
+      R *ODb = reinterpret_cast<R *>(A.od) + (long long)b * A.ods;
+      for (int base = e0; base < e1; base += KE * BLOCK) {
+        int l[KE][N], ee[KE];
+        R dev[KE];
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+          const int p = min(base + k * BLOCK + tid, e1 - 1);
+          load_lconn<N>(A.lconn + (long long)p * N, l[k]);
+          ee[k] = A.perm[p];
+        }
+        if (elemental) {
+#pragma unroll
+          for (int k = 0; k < KE; ++k) dev[k] = Fb[ee[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < KE; ++k) {
+          const int pos = base + k * BLOCK + tid - e0;
+          if (pos >= ne) continue;
+          E el;
+          R xr[N][3];
+#pragma unroll
+          for (int a = 0; a < N; ++a) {
+            if constexpr (sizeof(R) == 4) {
+              const float4 v = reinterpret_cast<const float4 *>(X)[l[k][a]];
+              xr[a][0] = v.x;
+              xr[a][1] = v.y;
+              xr[a][2] = v.z;
+            } else {
+              const double2 v0 = reinterpret_cast<const double2 *>(X)[2 * l[k][a]];
+              const double2 v1 = reinterpret_cast<const double2 *>(X)[2 * l[k][a] + 1];
+              xr[a][0] = v0.x;
+              xr[a][1] = v0.y;
+              xr[a][2] = v1.x;
+            }
+          }
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+#pragma unroll
+            for (int i = 0; i < D; ++i) el.x[a][i] = xr[a][i] - xr[0][i];
+          el.elemental = elemental;
+          if (elemental) {
+            el.de = dev[k];
+#pragma unroll
+            for (int a = 0; a < N; ++a) el.d[a] = dev[k];
+          } else {
+            el.de = R(0);
+#pragma unroll
+            for (int a = 0; a < N; ++a) el.d[a] = FD[l[k][a]];
+          }
+#pragma unroll
+          for (int a = 0; a < N; ++a) {
+            if constexpr (P == 3 && sizeof(R) == 4) {
+              const float4 v = reinterpret_cast<const float4 *>(U)[l[k][a]];
+              el.u[a * 3 + 0] = v.x;
+              el.u[a * 3 + 1] = v.y;
+              el.u[a * 3 + 2] = v.z;
+            } else {
+#pragma unroll
+              for (int c = 0; c < P; ++c) el.u[a * P + c] = U[l[k][a] * PP + c];
+            }
+          }
+          R ro[NL], gd[N];
+          if constexpr (MODE == MODE_RES) {
+            R energy;
+            elem_residual<R, ET, PH, NGP>(el, A.pp, ro, energy);
+            en += (double)energy;
+          } else if constexpr (MODE == MODE_VJP) {
+            R w[NL];
+#pragma unroll
+            for (int a = 0; a < N; ++a)
+#pragma unroll
+              for (int c = 0; c < P; ++c) w[a * P + c] = W[l[k][a] * PP + c];
+            elem_vjp<R, ET, PH, NGP>(el, w, A.pp, picard, ro, gd);
+          } else {
+            R pi;
+            elem_egrad<R, ET, PH, NGP>(el, A.pp, ro, gd, pi);
+            en += (double)pi;
+          }
+          if (want_ou) {  // corner results to LDS: one 16-byte store per corner when P == 3
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+              if constexpr (P == 3 && sizeof(R) == 4) {
+                reinterpret_cast<float4 *>(ER)[pos * N + a] = make_float4(ro[a * 3], ro[a * 3 + 1], ro[a * 3 + 2], 0.f);
+              } else {
+#pragma unroll
+                for (int c = 0; c < P; ++c) ER[(pos * N + a) * PP + c] = ro[a * P + c];
+              }
+            }
+          }
+          if constexpr (MODE != MODE_RES) {
+            if (want_gd) {
+              if (elemental) {
+                if (accumulate)
+                  unsafeAtomicAdd(&ODb[ee[k]], gd[0]);  // RMW the compiler cannot hoist as a load
+                else
+                  ODb[ee[k]] = gd[0];
+              } else {
+#pragma unroll
+                for (int a = 0; a < N; ++a) ERD[pos * N + a] = gd[a];
+              }
+            }
+          }
+        }
+      }
+      if (A.chunk_energy) {
+        en = wave_sum(en);
+        if ((tid & 63) == 0) red[tid >> 6] = en;
+      }
+      __syncthreads();
+      if (A.chunk_energy && tid == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < BLOCK / 64; ++k) s += red[k];
+        A.chunk_energy[(long long)b * A.n_chunks + chunk] = s;
+      }
+      // node-centric gather of the corner results (fixed order) and write-out
+      R *OU = reinterpret_cast<R *>(A.ou) + (long long)b * A.ous;
+      R *PU = reinterpret_cast<R *>(A.part_u) + (long long)b * A.n_slots * P;
+      R *PD = reinterpret_cast<R *>(A.part_d) + (long long)b * A.n_slots;
+      if (want_ou || nodal_gd) {
+        for (int base = 0; base < nl; base += KN * BLOCK) {
+          int g[KN], sl[KN], js[KN], je[KN];
+#pragma unroll
+          for (int k = 0; k < KN; ++k) {
+            const int i = min(base + k * BLOCK + tid, nl - 1);
+            g[k] = A.nodes[n0 + i] & 0x7fffffff;
+            sl[k] = A.node_slot[n0 + i];
+            js[k] = A.inc_start[n0 + i];
+            je[k] = (i + 1 < nl) ? (int)A.inc_start[n0 + i + 1] : N * ne;
+          }
+#pragma unroll
+          for (int k = 0; k < KN; ++k) {
+            const int i = base + k * BLOCK + tid;
+            if (i >= nl) continue;
+            if (want_ou) {
+              R s[P];
+#pragma unroll
+              for (int c = 0; c < P; ++c) s[c] = R(0);
+              for (int j = js[k]; j < je[k]; ++j) {
+                const int q = INC[j];
+                if constexpr (P == 3 && sizeof(R) == 4) {
+                  const float4 v = reinterpret_cast<const float4 *>(ER)[q];
+                  s[0] += v.x;
+                  s[1] += v.y;
+                  s[2] += v.z;
+                } else {
+#pragma unroll
+                  for (int c = 0; c < P; ++c) s[c] += ER[q * PP + c];
+                }
+              }
+#pragma unroll
+              for (int c = 0; c < P; ++c) {
+                if (sl[k] >= 0) {
+                  PU[(long long)sl[k] * P + c] = s[c];
+                } else {
+                  const long long dof = (long long)g[k] * P + c;
+                  bool masked = false;
+                  if (mask) masked = A.dmask[dof];
+                  if (accumulate) {
+                    if (!masked) unsafeAtomicAdd(OU + dof, s[c]);
+                  } else {
+                    OU[dof] = masked ? R(0) : s[c];
+                  }
+                }
+              }
+            }
+            if (nodal_gd) {
+              R s = R(0);
+              for (int j = js[k]; j < je[k]; ++j) s += ERD[INC[j]];
+              if (sl[k] >= 0)
+                PD[sl[k]] = s;
+              else if (accumulate)
+                unsafeAtomicAdd(&ODb[g[k]], s);
+              else
+                ODb[g[k]] = s;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <class R, int ET, int PH, int NGP, int MODE>
+static int launch_chunk_block(int block, dim3 grid, size_t shm, hipStream_t s, const ChunkArgs &A) {
+  if (block == 128) {
+    auto k = k_chunk<R, ET, PH, NGP, MODE, 128>;
+    hipLaunchKernelGGL(k, grid, dim3(128), shm, s, A);
+  } else if (block == 512) {
+    auto k = k_chunk<R, ET, PH, NGP, MODE, 512>;
+    hipLaunchKernelGGL(k, grid, dim3(512), shm, s, A);
+  } else {
+    auto k = k_chunk<R, ET, PH, NGP, MODE, 256>;
+    hipLaunchKernelGGL(k, grid, dim3(256), shm, s, A);
+  }
+  FH_HIP(hipGetLastError());
+  return FEMHOT_OK;
+}
+
+template <int ET>
+int launch_chunk_et_impl(int dtype, int ph, int ngp, int mode, int block, dim3 grid, size_t shm, hipStream_t s,
+                           const ChunkArgs &A) {
+  return dispatch_r_ph_ngp(dtype, ph, ngp, [&](auto rp, auto phc, auto ngpc) -> int {
+    using R = std::remove_pointer_t<decltype(rp)>;
+    constexpr int PH = decltype(phc)::value, NGP = decltype(ngpc)::value;
+    if (mode == MODE_RES) return launch_chunk_block<R, ET, PH, NGP, MODE_RES>(block, grid, shm, s, A);
+    if (mode == MODE_VJP) return launch_chunk_block<R, ET, PH, NGP, MODE_VJP>(block, grid, shm, s, A);
+    return launch_chunk_block<R, ET, PH, NGP, MODE_EGRAD>(block, grid, shm, s, A);
+  });
+}
+
+template <int ET>
+int launch_chunk_et(int dtype, int ph, int ngp, int mode, int block, dim3 grid, size_t shm, hipStream_t s,
+                    const ChunkArgs &A);
+
+}  // namespace femhot

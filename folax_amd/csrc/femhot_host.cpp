@@ -168,6 +168,8 @@ static void free_chunks(ChunkPlan &p) {
   (void)hipFree(p.red_off);
   (void)hipFree(p.red_slots);
   (void)hipFree(p.seg_chunk_off);
+  (void)hipFree(p.inc);
+  (void)hipFree(p.inc_start);
   p = ChunkPlan{};
 }
 
@@ -220,7 +222,8 @@ int ensure_chunks(Ctx *c, int epc) {
     return sx != sy ? sx < sy : x.first < y.first;
   });
   std::vector<int> perm(Ne), elem_off{0}, node_off{0}, seg, nodes;
-  std::vector<uint16_t> lconn((size_t)Ne * n);
+  std::vector<uint16_t> lconn((size_t)Ne * n), inc((size_t)Ne * n), inc_start;
+  std::vector<int> cnt;
   std::vector<float> xyz32;
   std::vector<double> xyz64;
   for (long long i = 0; i < Ne; ++i) perm[i] = key[i].second;
@@ -241,15 +244,30 @@ int ensure_chunks(Ctx *c, int epc) {
       set_error("ensure_chunks: more than 65535 nodes in a chunk");
       return FEMHOT_ERR_UNSUPPORTED;
     }
+    if ((j - i) * n > 65535) {
+      set_error("ensure_chunks: too many element corners in a chunk");
+      return FEMHOT_ERR_UNSUPPORTED;
+    }
     for (long long k = i; k < j; ++k)
       for (int a = 0; a < n; ++a) {
         const int g = c->h_conn[(size_t)perm[k] * n + a];
         lconn[(size_t)k * n + a] = (uint16_t)(std::lower_bound(loc.begin(), loc.end(), g) - loc.begin());
       }
+    // incidence (counting sort by local node; entries of a node in element-position order)
+    cnt.assign(loc.size() + 1, 0);
+    for (long long k = i; k < j; ++k)
+      for (int a = 0; a < n; ++a) cnt[lconn[(size_t)k * n + a] + 1]++;
+    for (size_t q = 0; q < loc.size(); ++q) cnt[q + 1] += cnt[q];
+    for (size_t q = 0; q < loc.size(); ++q) inc_start.push_back((uint16_t)cnt[q]);
+    for (long long k = i; k < j; ++k)
+      for (int a = 0; a < n; ++a) {
+        const int q = lconn[(size_t)k * n + a];
+        inc[(size_t)i * n + cnt[q]++] = (uint16_t)((k - i) * n + a);
+      }
     const double *ref = &c->h_coords[(size_t)loc[0] * 3];
     for (int g : loc)
-      for (int k = 0; k < 3; ++k) {
-        const double v = c->h_coords[(size_t)g * 3 + k] - ref[k];
+      for (int k = 0; k < 4; ++k) {  // 16-byte (32-byte for FP64) record per node: x, y, z, 0
+        const double v = k < 3 ? c->h_coords[(size_t)g * 3 + k] - ref[k] : 0.0;
         xyz64.push_back(v);
         xyz32.push_back((float)v);
       }
@@ -308,7 +326,8 @@ int ensure_chunks(Ctx *c, int epc) {
       (rc = upload(&p.chunk_xyz64, xyz64)) || (rc = upload(&p.zero_nodes, zero)) ||
       (rc = upload(&p.node_slot, node_slot)) || (rc = upload(&p.sh_nodes, sh_nodes)) ||
       (rc = upload(&p.red_off, red_off)) || (rc = upload(&p.red_slots, red_slots)) ||
-      (rc = upload(&p.seg_chunk_off, seg_chunk_off))) {
+      (rc = upload(&p.seg_chunk_off, seg_chunk_off)) || (rc = upload(&p.inc, inc)) ||
+      (rc = upload(&p.inc_start, inc_start))) {
     free_chunks(p);
     return rc;
   }

@@ -47,7 +47,7 @@ struct ChunkPlan {
   int *perm = nullptr;            // (Ne) original element id at each position
   uint16_t *lconn = nullptr;      // (Ne * n) chunk-local node indices, position order
   int *chunk_nodes = nullptr;     // global node id | 0x80000000 when shared
-  float *chunk_xyz32 = nullptr;   // per chunk node: coordinates relative to the chunk's first node
+  float *chunk_xyz32 = nullptr;   // per chunk node: (x, y, z, 0) relative to the chunk's first node
   double *chunk_xyz64 = nullptr;
   long long n_zero = 0;           // nodes in no element (their outputs are cleared separately)
   int *zero_nodes = nullptr;
@@ -58,6 +58,10 @@ struct ChunkPlan {
   int *red_off = nullptr;         // (n_shared+1) offsets into red_slots
   int *red_slots = nullptr;       // (n_slots) slots of each shared node, in chunk order
   int *seg_chunk_off = nullptr;   // (nseg+1) chunks of each segment (chunks are segment-sorted)
+  // node-centric gather inside a chunk: for each chunk node, the (element position * n + corner)
+  // entries touching it, sorted by element position; per chunk the list has n * elements entries
+  uint16_t *inc = nullptr;        // (Ne * n), chunk by chunk
+  uint16_t *inc_start = nullptr;  // (chunk nodes) start of each node's entries within its chunk
 };
 
 // Device scratch of the chunk path (partials and per-chunk energies), grown on demand.

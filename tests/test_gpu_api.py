@@ -1,0 +1,246 @@
+"""GPU tests of the reference-shaped host API (folax_amd.loss_functions / solvers) against the
+reference's own goldens and the FP64 oracle."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import fe
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+G = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")))
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def t64(x):
+    return torch.as_tensor(np.asarray(x), dtype=torch.float64, device="cuda")
+
+
+def _sa_loss(N):
+    from folax_amd import loss_functions as lf
+    from folax_amd.mesh import create_2D_square_mesh
+    m = create_2D_square_mesh(L=1, N=N)
+    bc = {"Ux": {"left": 0.0, "right": 0.05}, "Uy": {"left": 0.0, "right": 0.05}}
+    loss = lf.MechanicalLoss2DQuad("mechanical_loss_2d", {"dirichlet_bc_dict": bc, "num_gp": 2,
+                                                          "material_dict": {"young_modulus": 1, "poisson_ratio": 0.3}}, m)
+    loss.Initialize()
+    return m, loss
+
+
+def _response_grads(coords, conn, K, u):
+    """Response (E**2)*U[0] (fe_response.py:73-113) gradients: test-side, not part of the hot path."""
+    N, w, DN, detJ = fe.geometry("quad", coords[conn], 2)
+    Eg = np.einsum("ga,ea->eg", N, K[conn])
+    dofs = fe.element_dofs(conn, 2)
+    U0 = np.einsum("ga,ea->eg", N, u[dofs][:, 0::2])
+    wd = w[None] * detJ
+    dfdu_e = np.zeros((conn.shape[0], 8))
+    dfdu_e[:, 0::2] = np.einsum("eg,ga->ea", wd * Eg ** 2, N)
+    dfdu = np.zeros(2 * coords.shape[0])
+    np.add.at(dfdu, dofs, dfdu_e)
+    dfdE = np.zeros(coords.shape[0])
+    np.add.at(dfdE, conn, np.einsum("eg,ga->ea", wd * 2 * Eg * U0, N))
+    return dfdu, dfdE
+
+
+def test_sensitivity_analysis_golden_through_api():
+    """tests/unit/test_sensitivity_analysis.py through the drop-in API."""
+    sa = G["tests/unit/test_sensitivity_analysis.py"]["test_quad"]
+    K = np.array(sa["assigns"]["random_K"])
+    u = np.array(sa["assigns"]["random_FE_UV"])
+    lam = np.array(sa["assigns"]["random_adj_FE_UV"])
+    m, loss = _sa_loss(3)
+    A, _ = loss.ComputeJacobianMatrixAndResidualVector(t64(K), t64(u), transpose_jacobian=True)
+    chk = sa["checks"][0]
+    np.testing.assert_allclose(A.todense()[8].cpu().numpy(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+    _, gd = loss.ComputeResidualVJP(t64(K), t64(u), t64(lam))
+    _, dfdE = _response_grads(m.GetNodesCoordinates(), m.GetElementsNodes("quad"), K, u)
+    chk = sa["checks"][2]
+    np.testing.assert_allclose(dfdE + gd.cpu().numpy(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+
+
+@pytest.mark.parametrize("solver", ["PCG", "PCG-mixed"])
+def test_mechanical_2D_sa_golden_through_api(solver):
+    """tests/integration/test_mechanical_2D_sa.py: linear solve -> adjoint solve -> control
+    derivatives, with the device PCG in place of the direct solver."""
+    from folax_amd.solvers import FiniteElementLinearResidualBasedSolver
+    sa = G["tests/integration/test_mechanical_2D_sa.py"]
+    K = np.array(sa["setUp"]["assigns"]["self.random_K"])
+    m, loss = _sa_loss(5)
+    s = FiniteElementLinearResidualBasedSolver("lin", loss, {"linear_solver_settings": {"solver": solver, "tol": 1e-12}})
+    s.Initialize()
+    u = s.Solve(t64(K), np.zeros(2 * m.GetNumberOfNodes()))
+    Kt, _ = loss.ComputeJacobianMatrixAndResidualVector(t64(K), u, transpose_jacobian=True)
+    coords, conn = m.GetNodesCoordinates(), m.GetElementsNodes("quad")
+    dfdu, dfdE = _response_grads(coords, conn, K, u.cpu().numpy())
+    rhs = dfdu.copy()
+    rhs[loss.dirichlet_indices] = 0.0
+    lam = s.LinearSolve(Kt, t64(rhs))  # solves Kt x = -rhs
+    _, gd = loss.ComputeResidualVJP(t64(K), u, lam)
+    chk = sa["test_sensitivites"]["checks"][0]
+    np.testing.assert_allclose(dfdE + gd.cpu().numpy(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+
+
+def _c1_inputs():
+    from folax_amd.controls import fourier_field
+    from folax_amd.mesh import create_2D_square_mesh
+    m = create_2D_square_mesh(1.0, 41)
+    k = fourier_field(m.GetNodesCoordinates(), np.random.default_rng(1001).standard_normal(16), vmin=0.1, vmax=1.0)
+    T = np.random.default_rng(1002).uniform(0.0, 1.0, m.GetNumberOfNodes())
+    return m, k, T
+
+
+def test_c1_thermal_quad4_40x40_fp64():
+    """2-D steady linear heat, 40x40 QUAD4, Fourier conductivity, T=1 left / 0 right: residual,
+    tangent and CG solve vs the oracle (direct solve on the same CSR)."""
+    from folax_amd import loss_functions as lf
+    from folax_amd.solvers import FiniteElementLinearResidualBasedSolver
+    m, k, T = _c1_inputs()
+    loss = lf.ThermalLoss2DQuad("thermal", {"dirichlet_bc_dict": {"T": {"left": 1.0, "right": 0.0}}}, m)
+    loss.Initialize()
+    coords, conn = m.GetNodesCoordinates(), m.GetElementsNodes("quad")
+    ph = fe.Physics("thermal", "quad", num_gp=2)
+    idx, val = fe.dirichlet_dofs(m.node_sets, ["T"], {"T": {"left": 1.0, "right": 0.0}})
+    Tb = fe.apply_dirichlet(T, idx, val)
+    A_ref, R_ref = fe.assemble(ph, coords, conn, k, Tb, idx)
+    A, R = loss.ComputeJacobianMatrixAndResidualVector(t64(k), loss.ApplyDirichletBCOnDofVector(t64(T)))
+    assert A.values.numel() == 14641 == A_ref.nnz
+    assert np.array_equal(A.row_ptr.cpu().numpy(), A_ref.indptr)
+    assert np.array_equal(A.col_idx.cpu().numpy(), A_ref.indices)
+    assert rel(A.values.cpu().numpy(), A_ref.data) <= 1e-12
+    assert rel(R.cpu().numpy(), R_ref) <= 1e-12
+    e = loss.ComputeTotalEnergy(t64(k), t64(Tb))
+    assert abs(float(e) - fe.total_energy(ph, coords, conn, k, Tb)) <= 1e-12 * abs(float(e))
+    u_ref = fe.linear_solve(ph, coords, conn, k, T, idx, val)
+    for solver in ("PCG", "PCG-mixed"):
+        s = FiniteElementLinearResidualBasedSolver("s", loss, {"linear_solver_settings": {"solver": solver,
+                                                                                          "tol": 1e-12}})
+        s.Initialize()
+        u = s.Solve(t64(k), t64(T))
+        assert rel(u.cpu().numpy(), u_ref) <= 1e-10, solver
+
+
+@pytest.mark.parametrize("kind", ["thermal", "mechanical", "neohooke"])
+@pytest.mark.parametrize("exponent", [1.0, 2.0])
+def test_batch_loss_value_and_autograd(kind, exponent):
+    """ComputeBatchLoss: value = mean_b (sum_e E_e)^p; gradients follow jax.grad of the reference
+    (stop_gradient semantics for thermal / linear elastic, full derivative for Neo-Hookean)."""
+    from folax_amd import loss_functions as lf
+    from folax_amd.mesh import create_3D_box_mesh
+    m = create_3D_box_mesh(3, 3, 3)
+    coords, conn = m.GetNodesCoordinates(), m.GetElementsNodes("hexahedron")
+    rng = np.random.default_rng(7)
+    B = 3
+    if kind == "thermal":
+        cls, dofs, bc = lf.ThermalLoss3DHexa, ["T"], {"T": {"left": 1.0, "right": 0.0}}
+        settings = {"dirichlet_bc_dict": bc}
+        ph = fe.Physics("thermal", "hexahedron")
+        U = rng.uniform(0, 1, size=(B, coords.shape[0]))
+    else:
+        cls = lf.MechanicalLoss3DHexa if kind == "mechanical" else lf.NeoHookeMechanicalLoss3DHexa
+        dofs, bc = ["Ux", "Uy", "Uz"], {"Ux": {"bottom": 0.0}, "Uy": {"bottom": 0.0}, "Uz": {"bottom": 0.0, "top": 0.01}}
+        settings = {"dirichlet_bc_dict": bc, "material_dict": {"young_modulus": 1.0, "poisson_ratio": 0.3}}
+        ph = fe.Physics(kind, "hexahedron", E=1.0, nu=0.3)
+        U = rng.uniform(-0.02, 0.02, size=(B, 3 * coords.shape[0]))
+    settings["loss_function_exponent"] = exponent
+    loss = cls("l", settings, m)
+    loss.Initialize()
+    Kf = rng.uniform(0.5, 1.5, size=(B, coords.shape[0]))
+    idx, val = fe.dirichlet_dofs(m.node_sets, dofs, bc)
+    mean_ref, (lo, hi, _) = fe.batch_loss(ph, coords, conn, Kf, U, idx, val, exponent)
+    Pt = t64(Kf).requires_grad_(True)
+    Ut = t64(U).requires_grad_(True)
+    mean, (mn, mx, _) = loss.ComputeBatchLoss(Pt, Ut)
+    assert abs(float(mean) - mean_ref) <= 1e-12 * abs(mean_ref)
+    assert abs(float(mn) - lo) <= 1e-12 * abs(lo) and abs(float(mx) - hi) <= 1e-12 * abs(hi)
+    mean.backward()
+    Ub = fe.apply_dirichlet(U, idx, val)
+    for b in range(B):
+        Eb = fe.total_energy(ph, coords, conn, Kf[b], Ub[b])
+        scale = exponent * Eb ** (exponent - 1.0) / B
+        if kind == "neohooke":
+            _, gu_ref, gd_ref = fe.global_energy_grad(ph, coords, conn, Kf[b], Ub[b])
+        else:
+            _, gu_ref = fe.assemble(ph, coords, conn, Kf[b], Ub[b], np.zeros(0, np.int64), want_K=False)
+            gd_ref = np.zeros(coords.shape[0])
+        gu_ref = gu_ref.copy()
+        gu_ref[idx] = 0.0
+        assert rel(Ut.grad[b].cpu().numpy(), scale * gu_ref) <= 1e-11
+        if kind == "neohooke":
+            assert rel(Pt.grad[b].cpu().numpy(), scale * gd_ref) <= 1e-11
+        else:
+            assert float(Pt.grad[b].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_residual_norm_loss_and_vjp(dtype):
+    """1/2 sum_b ||R_free||^2 for nonlinear thermal QUAD4 and its exact gradient (w.r.t. T and k0)."""
+    from folax_amd import loss_functions as lf
+    from folax_amd.mesh import create_2D_square_mesh
+    m = create_2D_square_mesh(1.0, 17)
+    coords, conn = m.GetNodesCoordinates(), m.GetElementsNodes("quad")
+    bc = {"T": {"left": 1.0, "right": 0.0}}
+    loss = lf.ThermalLoss2DQuad("t", {"dirichlet_bc_dict": bc, "beta": 0.5, "c": 2}, m)
+    loss.Initialize()
+    rng = np.random.default_rng(8)
+    B = 4
+    K0 = rng.uniform(0.1, 1.0, size=(B, coords.shape[0]))
+    T = rng.uniform(0, 1, size=(B, coords.shape[0]))
+    Pt = torch.as_tensor(K0, dtype=dtype, device="cuda").requires_grad_(True)
+    Tt = torch.as_tensor(T, dtype=dtype, device="cuda").requires_grad_(True)
+    L = loss.ComputeBatchResidualNormLoss(Pt, Tt)
+    L.backward()
+    ph = fe.Physics("thermal", "quad", num_gp=2, beta=0.5, c=2.0)
+    idx, val = fe.dirichlet_dofs(m.node_sets, ["T"], bc)
+    Lref, gus, gds = 0.0, [], []
+    for b in range(B):
+        Tb = fe.apply_dirichlet(T[b], idx, val)
+        l, gu, gd = fe.residual_norm_loss_grad(ph, coords, conn, K0[b], Tb, idx)
+        Lref += l
+        gus.append(gu)
+        gds.append(gd)
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    assert abs(float(L) - Lref) <= tol * abs(Lref)
+    assert rel(Tt.grad.cpu().numpy(), np.stack(gus)) <= tol
+    assert rel(Pt.grad.cpu().numpy(), np.stack(gds)) <= tol
+
+
+def test_nonlinear_solver_picard_thermal():
+    """Load-stepped Newton loop (fe_nonlinear_residual_based_solver.py:76-170) on nonlinear thermal,
+    whose tangent is the Picard matrix: same iteration as the oracle, same answer."""
+    from folax_amd import loss_functions as lf
+    from folax_amd.solvers import FiniteElementNonLinearResidualBasedSolver
+    m, k, T = _c1_inputs()
+    bc = {"T": {"left": 1.0, "right": 0.0}}
+    loss = lf.ThermalLoss2DQuad("t", {"dirichlet_bc_dict": bc, "beta": 0.5, "c": 2}, m)
+    loss.Initialize()
+    settings = {"linear_solver_settings": {"solver": "PCG", "tol": 1e-13},
+                "nonlinear_solver_settings": {"rel_tol": 1e-10, "abs_tol": 1e-10, "maxiter": 12, "load_incr": 2}}
+    s = FiniteElementNonLinearResidualBasedSolver("nl", loss, settings)
+    s.Initialize()
+    u = s.Solve(t64(k), np.zeros(m.GetNumberOfNodes())).cpu().numpy()
+    coords, conn = m.GetNodesCoordinates(), m.GetElementsNodes("quad")
+    ph = fe.Physics("thermal", "quad", num_gp=2, beta=0.5, c=2.0)
+    idx, val = fe.dirichlet_dofs(m.node_sets, ["T"], bc)
+    import scipy.sparse.linalg as spla
+    ur = np.zeros(coords.shape[0])
+    for step in (1, 2):
+        ur = fe.apply_dirichlet(ur, idx, val, step / 2)
+        for i in range(12):
+            A, R = fe.assemble(ph, coords, conn, k, ur, idx)
+            du = spla.spsolve(A.tocsc(), -R)
+            if np.linalg.norm(R) < 1e-10 or np.linalg.norm(du) < 1e-10 or i == 11:
+                break
+            ur = ur + du
+    assert rel(u, ur) <= 1e-9
+    # the converged field is a fixed point of the Picard iteration
+    _, R = fe.assemble(ph, coords, conn, k, u, idx, want_K=False)
+    assert np.linalg.norm(R) <= 1e-8
