@@ -59,8 +59,8 @@ template <int P> struct Pad { static constexpr int PP = (P == 3) ? 4 : P; };
 //   U   [cap4 * PP]     nodal solution
 //   W   [cap4 * PP]     cotangent (VJP only)
 //   FD  [cap4]          nodal field (nodal fields only)
-//   ER  [emax * N * PP] per element corner: residual / g_u contribution
-//   ERD [emax * N]      per element corner: nodal field-gradient contribution (VJP / EGRAD, nodal)
+//   ER  [N * emax * PP] per element corner, corner-major ([a][position]): residual / g_u contribution
+//   ERD [N * emax]      per element corner, corner-major: nodal field-gradient contribution
 //   INC [emax * N] (uint16) the chunk's incidence list
 template <class R, int ET, int PH, int NGP, int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
@@ -68,6 +68,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL, PP = Pad<P>::PP;
   constexpr int KN = 4;                                  // nodes per lane per round
   constexpr int KE = (ET != HEX8 && NGP <= 2) ? 4 : 1;  // elements per lane per round
+  constexpr int LOGN = N == 4 ? 2 : 3;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[BLOCK / 64];
   const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
@@ -104,8 +105,10 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
         }
       }
       if (want_ou || nodal_gd) {
-        const uint16_t *inc = A.inc + (long long)N * e0;
-        for (int j = tid; j < N * ne; j += BLOCK) INC[j] = inc[j];
+        // one element's worth (N entries, 8 or 16 bytes) per access
+        using UN = typename std::conditional<N == 4, uint2, uint4>::type;
+        const UN *inc = reinterpret_cast<const UN *>(A.inc + (long long)N * e0);
+        for (int j = tid; j < ne; j += BLOCK) reinterpret_cast<UN *>(INC)[j] = inc[j];
       }
     }
     const int b0 = sbi * A.sb, b1 = min(A.batch, b0 + A.sb);
@@ -222,14 +225,15 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             elem_egrad<R, ET, PH, NGP>(el, A.pp, ro, gd, pi);
             en += (double)pi;
           }
-          if (want_ou) {  // corner results to LDS: one 16-byte store per corner when P == 3
+          if (want_ou) {  // corner-major, one 16-byte store per corner when P == 3: consecutive lanes
+                          // write consecutive records
 #pragma unroll
             for (int a = 0; a < N; ++a) {
               if constexpr (P == 3 && sizeof(R) == 4) {
-                reinterpret_cast<float4 *>(ER)[pos * N + a] = make_float4(ro[a * 3], ro[a * 3 + 1], ro[a * 3 + 2], 0.f);
+                reinterpret_cast<float4 *>(ER)[a * emax + pos] = make_float4(ro[a * 3], ro[a * 3 + 1], ro[a * 3 + 2], 0.f);
               } else {
 #pragma unroll
-                for (int c = 0; c < P; ++c) ER[(pos * N + a) * PP + c] = ro[a * P + c];
+                for (int c = 0; c < P; ++c) ER[(a * emax + pos) * PP + c] = ro[a * P + c];
               }
             }
           }
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
                   ODb[ee[k]] = gd[0];
               } else {
 #pragma unroll
-                for (int a = 0; a < N; ++a) ERD[pos * N + a] = gd[a];
+                for (int a = 0; a < N; ++a) ERD[a * emax + pos] = gd[a];
               }
             }
           }
@@ -284,14 +288,15 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
               for (int c = 0; c < P; ++c) s[c] = R(0);
               for (int j = js[k]; j < je[k]; ++j) {
                 const int q = INC[j];
+                const int r = (q & (N - 1)) * emax + (q >> LOGN);
                 if constexpr (P == 3 && sizeof(R) == 4) {
-                  const float4 v = reinterpret_cast<const float4 *>(ER)[q];
+                  const float4 v = reinterpret_cast<const float4 *>(ER)[r];
                   s[0] += v.x;
                   s[1] += v.y;
                   s[2] += v.z;
                 } else {
 #pragma unroll
-                  for (int c = 0; c < P; ++c) s[c] += ER[q * PP + c];
+                  for (int c = 0; c < P; ++c) s[c] += ER[r * PP + c];
                 }
               }
 #pragma unroll
@@ -312,7 +317,10 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             }
             if (nodal_gd) {
               R s = R(0);
-              for (int j = js[k]; j < je[k]; ++j) s += ERD[INC[j]];
+              for (int j = js[k]; j < je[k]; ++j) {
+                const int q = INC[j];
+                s += ERD[(q & (N - 1)) * emax + (q >> LOGN)];
+              }
               if (sl[k] >= 0)
                 PD[sl[k]] = s;
               else if (accumulate)

@@ -122,6 +122,13 @@ __device__ __forceinline__ void shape(double x, double y, double z, R (&Nv)[Elem
   }
 }
 
+// 1/x: FP64 divides exactly; FP32 takes the hardware reciprocal (1 ulp) and one Newton step.
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+}
+
 template <class R, int ET> struct GP {
   static constexpr int N = ElemT<ET>::N, D = ElemT<ET>::D;
   R Nv[N];
@@ -137,21 +144,27 @@ __device__ __forceinline__ void gp_geometry(int g, const R (&x)[ElemT<ET>::N][El
   gauss_point<ET, NGP>(g, xi, eta, zeta, w);
   R dNl[N][D];
   shape<ET, R>(xi, eta, zeta, G.Nv, dNl);
+  // x is relative to local node 0 (x[0] == 0), so node 0 drops out of J; for TET4 dN/dxi is
+  // constant and J is the matrix of edge vectors x1, x2, x3.
   R J[D][D];
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      R s = R(0);
+      if constexpr (ET == TET4) {
+        J[i][k] = x[k + 1][i];
+      } else {
+        R s = x[1][i] * dNl[1][k];
 #pragma unroll
-      for (int a = 0; a < N; ++a) s += x[a][i] * dNl[a][k];
-      J[i][k] = s;
+        for (int a = 2; a < N; ++a) s += x[a][i] * dNl[a][k];
+        J[i][k] = s;
+      }
     }
   R inv[D][D];
   R det;
   if constexpr (D == 2) {
     det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const R id = R(1) / det;
+    const R id = recip(det);
     inv[0][0] = J[1][1] * id;
     inv[0][1] = -J[0][1] * id;
     inv[1][0] = -J[1][0] * id;
@@ -161,7 +174,7 @@ __device__ __forceinline__ void gp_geometry(int g, const R (&x)[ElemT<ET>::N][El
     const R c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
     const R c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
     det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    const R id = R(1) / det;
+    const R id = recip(det);
     inv[0][0] = c00 * id;
     inv[1][0] = c01 * id;
     inv[2][0] = c02 * id;
@@ -172,15 +185,25 @@ __device__ __forceinline__ void gp_geometry(int g, const R (&x)[ElemT<ET>::N][El
     inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
     inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
   }
-#pragma unroll
-  for (int a = 0; a < N; ++a)
+  if constexpr (ET == TET4) {  // dN_a/dx = row a-1 of inv(J) for a > 0; node 0 takes minus their sum
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      R s = R(0);
-#pragma unroll
-      for (int k = 0; k < D; ++k) s += dNl[a][k] * inv[k][j];
-      G.DN[a][j] = s;
+      G.DN[1][j] = inv[0][j];
+      G.DN[2][j] = inv[1][j];
+      G.DN[3][j] = inv[2][j];
+      G.DN[0][j] = -((inv[0][j] + inv[1][j]) + inv[2][j]);
     }
+  } else {
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        R s = dNl[a][0] * inv[0][j];
+#pragma unroll
+        for (int k = 1; k < D; ++k) s += dNl[a][k] * inv[k][j];
+        G.DN[a][j] = s;
+      }
+  }
   G.wdet = R(w) * det;
 }
 
@@ -220,7 +243,7 @@ template <class R> __device__ __forceinline__ R xm_log1p(R x) {
     const R s = x / (R(2) + x);
     const R s2 = s * s;
     constexpr int K = sizeof(R) == 8 ? 16 : 8;
-    R t = R(0);
+    R t = R(-0.0);
 #pragma unroll
     for (int k = K; k >= 1; --k) t = t * s2 + R(1) / R(2 * k + 1);
     return x * x / (R(2) + x) - R(2) * s * s2 * t;
@@ -232,7 +255,7 @@ template <class R> __device__ __forceinline__ R xm_log1p(R x) {
 template <class R> __device__ __forceinline__ R expm1_mx(R x) {
   if (fabs(x) < R(0.5)) {
     constexpr int K = sizeof(R) == 8 ? 17 : 10;
-    R t = R(0);
+    R t = R(-0.0);
     // sum_{k>=2} x^k / k!  = x^2 (1/2! + x/3! + ...)
 #pragma unroll
     for (int k = K; k >= 2; --k) {
@@ -252,6 +275,7 @@ template <class R> __device__ __forceinline__ R expm1_mx(R x) {
 struct PhysParams {
   double E, nu, beta, c;
   double body[3];
+  int has_body;  // any body[k] != 0 (the body-force terms are skipped otherwise)
   // derived on the host
   double c2, c3, c4;      // 3-D D matrix entries for young_modulus E (mechanical.py:141-160)
   double f2, f2nu, f2sh;  // 2-D plane stress entries (mechanical.py:138-139)
@@ -300,7 +324,7 @@ __device__ __forceinline__ void nh_state(const R (&H)[D][D], R kap, R mu, NHStat
   const R L = dlog1p(delta);
   // E2 = C - I = H + H^T + H^T H
   R E2[D][D];
-  R hh = R(0);
+  R hh = R(-0.0);
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -311,7 +335,7 @@ __device__ __forceinline__ void nh_state(const R (&H)[D][D], R kap, R mu, NHStat
       E2[i][j] = s;
       hh += H[i][j] * H[i][j];
     }
-  R t = R(0);
+  R t = R(-0.0);
 #pragma unroll
   for (int i = 0; i < D; ++i) t += E2[i][i];
   const R J = R(1) + delta;
@@ -344,7 +368,7 @@ __device__ __forceinline__ void nh_state(const R (&H)[D][D], R kap, R mu, NHStat
   for (int i = 0; i < D; ++i)
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      R s = R(0);
+      R s = R(-0.0);
 #pragma unroll
       for (int k = 0; k < D; ++k) s -= st.invC[i][k] * E2[k][j];
       Y[i][j] = s;
@@ -385,12 +409,12 @@ __device__ __forceinline__ void nh_tangent_apply(const NHState<R, D> &st, R kap,
   for (int i = 0; i < D; ++i)
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      R s = R(0);
+      R s = R(-0.0);
 #pragma unroll
       for (int k = 0; k < D; ++k) s += st.invC[i][k] * X[k][j];
       IX[i][j] = s;
     }
-  R tr = R(0), sx = R(0);
+  R tr = R(-0.0), sx = R(-0.0);
 #pragma unroll
   for (int i = 0; i < D; ++i) tr += IX[i][i];
 #pragma unroll
@@ -405,7 +429,7 @@ __device__ __forceinline__ void nh_tangent_apply(const NHState<R, D> &st, R kap,
   for (int i = 0; i < D; ++i)
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      R A = R(0);  // (invC X invC)_ij
+      R A = R(-0.0);  // (invC X invC)_ij
 #pragma unroll
       for (int k = 0; k < D; ++k) A += IX[i][k] * st.invC[k][j];
       dS[i][j] = c1 * st.invC[i][j] * tr - c2 * A + c3 * (A - st.invC[i][j] * tr / R(D)) -

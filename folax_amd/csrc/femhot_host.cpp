@@ -469,6 +469,7 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
   p.beta = d->beta;
   p.c = d->c;
   for (int k = 0; k < 3; ++k) p.body[k] = d->body_force[k];
+  p.has_body = (p.body[0] != 0.0 || p.body[1] != 0.0 || p.body[2] != 0.0) ? 1 : 0;
   const double E = p.E, nu = p.nu;
   const double c1 = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
   p.c2 = c1 * (1.0 - nu);

@@ -26,7 +26,7 @@ template <class R, int ET, int PH> struct Elem {
 
 template <class R, int N> __device__ __forceinline__ R field_gp(const R (&d)[N], R de, bool elemental, const R (&Nv)[N]) {
   if (elemental) return de;
-  R s = R(0);
+  R s = R(-0.0);
 #pragma unroll
   for (int a = 0; a < N; ++a) s += Nv[a] * d[a];
   return s;
@@ -39,7 +39,7 @@ __device__ __forceinline__ void nodal_grad(const R (&v)[N * D], const R (&DN)[N]
   for (int i = 0; i < D; ++i)
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      R s = R(0);
+      R s = R(-0.0);
 #pragma unroll
       for (int a = 0; a < N; ++a) s += v[a * D + i] * DN[a][j];
       G[i][j] = s;
@@ -69,7 +69,7 @@ __device__ __forceinline__ void lin_stress(const R (&G)[D][D], const PhysParams 
 }
 
 template <class R, int D> __device__ __forceinline__ R ddot(const R (&A)[D][D], const R (&B)[D][D]) {
-  R s = R(0);
+  R s = R(-0.0);
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -84,7 +84,7 @@ __device__ __forceinline__ void add_nodal_force(const R (&S)[D][D], const R (&DN
   for (int a = 0; a < N; ++a)
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      R s = R(0);
+      R s = R(-0.0);
 #pragma unroll
       for (int j = 0; j < D; ++j) s += S[i][j] * DN[a][j];
       r[a * D + i] += f * s;
@@ -105,30 +105,30 @@ __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const P
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, G = RuleSize<ET, NGP>::G, UNR = G <= 8 ? G : 1;
 #pragma unroll
-  for (int i = 0; i < E::NL; ++i) r[i] = R(0);
-  R en = R(0);
+  for (int i = 0; i < E::NL; ++i) r[i] = R(-0.0);
+  R en = R(-0.0);
 #pragma unroll UNR
   for (int g = 0; g < G; ++g) {
     GP<R, ET> gp;
     gp_geometry<R, ET, NGP>(g, el.x, gp);
     const R eg = field_gp<R, N>(el.d, el.de, el.elemental, gp.Nv);
     if constexpr (PH == THERMAL) {
-      R Tg = R(0), gT[D];
+      R Tg = R(-0.0), gT[D];
 #pragma unroll
       for (int a = 0; a < N; ++a) Tg += gp.Nv[a] * el.u[a];
 #pragma unroll
       for (int j = 0; j < D; ++j) {
-        R s = R(0);
+        R s = R(-0.0);
 #pragma unroll
         for (int a = 0; a < N; ++a) s += gp.DN[a][j] * el.u[a];
         gT[j] = s;
       }
       const R k = eg * (R(1) + R(pp.beta) * powc(Tg, R(pp.c)));
       const R f = gp.wdet * k;
-      const R b = gp.wdet * R(pp.body[0]);
+      const R b = pp.has_body ? gp.wdet * R(pp.body[0]) : R(-0.0);
 #pragma unroll
       for (int a = 0; a < N; ++a) {
-        R s = R(0);
+        R s = R(-0.0);
 #pragma unroll
         for (int j = 0; j < D; ++j) s += gp.DN[a][j] * gT[j];
         r[a] += f * s - b * gp.Nv[a];
@@ -138,10 +138,12 @@ __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const P
       nodal_grad<R, N, D>(el.u, gp.DN, Gu);
       lin_stress<R, D>(Gu, pp, S);
       add_nodal_force<R, N, D>(S, gp.DN, gp.wdet * eg, r);
+      if (pp.has_body) {
 #pragma unroll
-      for (int a = 0; a < N; ++a)
+        for (int a = 0; a < N; ++a)
 #pragma unroll
-        for (int i = 0; i < D; ++i) r[a * D + i] -= gp.wdet * gp.Nv[a] * R(pp.body[i]);
+          for (int i = 0; i < D; ++i) r[a * D + i] -= gp.wdet * gp.Nv[a] * R(pp.body[i]);
+      }
     } else {
       R H[D][D];
       nodal_grad<R, N, D>(el.u, gp.DN, H);
@@ -153,16 +155,18 @@ __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const P
       for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          R s = R(0);
+          R s = R(-0.0);
 #pragma unroll
           for (int k = 0; k < D; ++k) s += st.F[i][k] * st.S[k][j];
           Pm[i][j] = s;
         }
       add_nodal_force<R, N, D>(Pm, gp.DN, gp.wdet * eg, r);
+      if (pp.has_body) {
 #pragma unroll
-      for (int a = 0; a < N; ++a)
+        for (int a = 0; a < N; ++a)
 #pragma unroll
-        for (int i = 0; i < D; ++i) r[a * D + i] -= gp.wdet * gp.Nv[a] * R(pp.body[i]);
+          for (int i = 0; i < D; ++i) r[a * D + i] -= gp.wdet * gp.Nv[a] * R(pp.body[i]);
+      }
       en += gp.wdet * eg * st.psi;
     }
   }
@@ -183,9 +187,9 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, G = RuleSize<ET, NGP>::G, UNR = G <= 8 ? G : 1;
 #pragma unroll
-  for (int i = 0; i < E::NL; ++i) gu[i] = R(0);
+  for (int i = 0; i < E::NL; ++i) gu[i] = R(-0.0);
 #pragma unroll
-  for (int a = 0; a < N; ++a) gd[a] = R(0);
+  for (int a = 0; a < N; ++a) gd[a] = R(-0.0);
 #pragma unroll UNR
   for (int g = 0; g < G; ++g) {
     GP<R, ET> gp;
@@ -193,12 +197,12 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
     const R eg = field_gp<R, N>(el.d, el.de, el.elemental, gp.Nv);
     R q;  // d(field at gp) weight for g_d
     if constexpr (PH == THERMAL) {
-      R Tg = R(0), gT[D], gW[D];
+      R Tg = R(-0.0), gT[D], gW[D];
 #pragma unroll
       for (int a = 0; a < N; ++a) Tg += gp.Nv[a] * el.u[a];
 #pragma unroll
       for (int j = 0; j < D; ++j) {
-        R s = R(0), t = R(0);
+        R s = R(0), t = R(-0.0);
 #pragma unroll
         for (int a = 0; a < N; ++a) {
           s += gp.DN[a][j] * el.u[a];
@@ -207,7 +211,7 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
         gT[j] = s;
         gW[j] = t;
       }
-      R wGT = R(0);
+      R wGT = R(-0.0);
 #pragma unroll
       for (int j = 0; j < D; ++j) wGT += gW[j] * gT[j];
       wGT *= gp.wdet;  // w^T G_g T
@@ -216,7 +220,7 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
       const R f = gp.wdet * eg * fac;
 #pragma unroll
       for (int a = 0; a < N; ++a) {
-        R s = R(0);
+        R s = R(-0.0);
 #pragma unroll
         for (int j = 0; j < D; ++j) s += gp.DN[a][j] * gW[j];
         gu[a] += f * s;
@@ -247,7 +251,7 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
       for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          R s = R(0);
+          R s = R(-0.0);
 #pragma unroll
           for (int k = 0; k < D; ++k) s += st.F[k][i] * dF[k][j] + st.F[k][j] * dF[k][i];
           X[i][j] = R(0.5) * s;
@@ -260,7 +264,7 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
       for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          R s = R(0);
+          R s = R(-0.0);
 #pragma unroll
           for (int k = 0; k < D; ++k) s += st.F[i][k] * dS[k][j] + dF[i][k] * st.S[k][j];
           M[i][j] = s;
@@ -286,10 +290,10 @@ __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const Phys
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, G = RuleSize<ET, NGP>::G, UNR = G <= 8 ? G : 1;
 #pragma unroll
-  for (int i = 0; i < E::NL; ++i) r[i] = R(0);
+  for (int i = 0; i < E::NL; ++i) r[i] = R(-0.0);
 #pragma unroll
-  for (int a = 0; a < N; ++a) gd[a] = R(0);
-  R pi = R(0);
+  for (int a = 0; a < N; ++a) gd[a] = R(-0.0);
+  R pi = R(-0.0);
 #pragma unroll UNR
   for (int g = 0; g < G; ++g) {
     GP<R, ET> gp;
@@ -300,20 +304,20 @@ __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const Phys
       R gT[D];
 #pragma unroll
       for (int j = 0; j < D; ++j) {
-        R s = R(0);
+        R s = R(-0.0);
 #pragma unroll
         for (int a = 0; a < N; ++a) s += gp.DN[a][j] * el.u[a];
         gT[j] = s;
       }
-      R g2 = R(0);
+      R g2 = R(-0.0);
 #pragma unroll
       for (int j = 0; j < D; ++j) g2 += gT[j] * gT[j];
       const R f = gp.wdet * eg;
-      const R b = gp.wdet * R(pp.body[0]);
-      R Tg = R(0);
+      const R b = pp.has_body ? gp.wdet * R(pp.body[0]) : R(-0.0);
+      R Tg = R(-0.0);
 #pragma unroll
       for (int a = 0; a < N; ++a) {
-        R s = R(0);
+        R s = R(-0.0);
 #pragma unroll
         for (int j = 0; j < D; ++j) s += gp.DN[a][j] * gT[j];
         r[a] += f * s - b * gp.Nv[a];
@@ -327,15 +331,17 @@ __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const Phys
       lin_stress<R, D>(Gu, pp, S);
       add_nodal_force<R, N, D>(S, gp.DN, gp.wdet * eg, r);
       q = R(0.5) * gp.wdet * ddot<R, D>(Gu, S);
-      R work = R(0);
+      R work = R(-0.0);
+      if (pp.has_body) {
 #pragma unroll
-      for (int a = 0; a < N; ++a)
+        for (int a = 0; a < N; ++a)
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-          const R fb = gp.wdet * gp.Nv[a] * R(pp.body[i]);
-          r[a * D + i] -= fb;
-          work += fb * el.u[a * D + i];
-        }
+          for (int i = 0; i < D; ++i) {
+            const R fb = gp.wdet * gp.Nv[a] * R(pp.body[i]);
+            r[a * D + i] -= fb;
+            work += fb * el.u[a * D + i];
+          }
+      }
       pi += eg * q - work;
     } else {
       R H[D][D];
@@ -347,22 +353,24 @@ __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const Phys
       for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          R s = R(0);
+          R s = R(-0.0);
 #pragma unroll
           for (int k = 0; k < D; ++k) s += st.F[i][k] * st.S[k][j];
           Pm[i][j] = s;
         }
       add_nodal_force<R, N, D>(Pm, gp.DN, gp.wdet * eg, r);
       q = gp.wdet * st.psi;
-      R work = R(0);
+      R work = R(-0.0);
+      if (pp.has_body) {
 #pragma unroll
-      for (int a = 0; a < N; ++a)
+        for (int a = 0; a < N; ++a)
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-          const R fb = gp.wdet * gp.Nv[a] * R(pp.body[i]);
-          r[a * D + i] -= fb;
-          work += fb * el.u[a * D + i];
-        }
+          for (int i = 0; i < D; ++i) {
+            const R fb = gp.wdet * gp.Nv[a] * R(pp.body[i]);
+            r[a * D + i] -= fb;
+            work += fb * el.u[a * D + i];
+          }
+      }
       pi += eg * q - work;
     }
     if (el.elemental)
@@ -386,7 +394,7 @@ __device__ __forceinline__ void elem_tangent_rows(const Elem<R, ET, PH> &el, con
 #pragma unroll
   for (int c = 0; c < P; ++c)
 #pragma unroll
-    for (int i = 0; i < E::NL; ++i) K[c][i] = R(0);
+    for (int i = 0; i < E::NL; ++i) K[c][i] = R(-0.0);
 #pragma unroll UNR
   for (int g = 0; g < G; ++g) {
     GP<R, ET> gp;
@@ -395,19 +403,19 @@ __device__ __forceinline__ void elem_tangent_rows(const Elem<R, ET, PH> &el, con
     R dNa[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      R s = R(0);
+      R s = R(-0.0);
 #pragma unroll
       for (int b = 0; b < N; ++b) s += (b == a) ? gp.DN[b][j] : R(0);
       dNa[j] = s;
     }
     if constexpr (PH == THERMAL) {
-      R Tg = R(0);
+      R Tg = R(-0.0);
 #pragma unroll
       for (int b = 0; b < N; ++b) Tg += gp.Nv[b] * el.u[b];
       const R f = gp.wdet * eg * (R(1) + R(pp.beta) * powc(Tg, R(pp.c)));
 #pragma unroll
       for (int b = 0; b < N; ++b) {
-        R s = R(0);
+        R s = R(-0.0);
 #pragma unroll
         for (int j = 0; j < D; ++j) s += dNa[j] * gp.DN[b][j];
         K[0][b] += f * s;
@@ -424,7 +432,7 @@ __device__ __forceinline__ void elem_tangent_rows(const Elem<R, ET, PH> &el, con
         lin_stress<R, D>(Gc, pp, S);
         R row[E::NL];
 #pragma unroll
-        for (int i = 0; i < E::NL; ++i) row[i] = R(0);
+        for (int i = 0; i < E::NL; ++i) row[i] = R(-0.0);
         add_nodal_force<R, N, D>(S, gp.DN, f, row);
 #pragma unroll
         for (int i = 0; i < E::NL; ++i) K[c][i] += row[i];
@@ -438,7 +446,7 @@ __device__ __forceinline__ void elem_tangent_rows(const Elem<R, ET, PH> &el, con
       R SdNa[D];  // S dN_a
 #pragma unroll
       for (int i = 0; i < D; ++i) {
-        R s = R(0);
+        R s = R(-0.0);
 #pragma unroll
         for (int k = 0; k < D; ++k) s += st.S[i][k] * dNa[k];
         SdNa[i] = s;
@@ -465,7 +473,7 @@ __device__ __forceinline__ void elem_tangent_rows(const Elem<R, ET, PH> &el, con
           }
         R row[E::NL];
 #pragma unroll
-        for (int i = 0; i < E::NL; ++i) row[i] = R(0);
+        for (int i = 0; i < E::NL; ++i) row[i] = R(-0.0);
         add_nodal_force<R, N, D>(M, gp.DN, f, row);
 #pragma unroll
         for (int i = 0; i < E::NL; ++i) K[c][i] += row[i];
