@@ -3,15 +3,17 @@
 //
 // One workgroup owns one chunk of spatially compact elements at a time (grid-stride loop over
 // chunk x sample-block work items, so the launch can be persistent).  Per work item:
-//   1. stage the chunk's node coordinates (precomputed chunk-relative, contiguous -> coalesced)
-//      and gather the chunk's nodal solution / cotangent / field (node ids sorted) into LDS;
-//   2. one lane per element: element routine reading LDS, nodal results accumulated with LDS
-//      float atomics (ds_add), per-element results written straight to HBM;
-//   3. every chunk node is written once with a plain store: to the output when the node belongs
-//      to this chunk only, to the node's partial slot when it is shared with other chunks.
-// k_reduce_shared then sums each shared node's partials in fixed chunk order and k_reduce_energy
-// sums per-chunk energies per segment, so the path uses no global atomics (device-scope atomics
-// resolve beyond the per-XCD L2) and its results are bitwise reproducible run to run.
+//   1. stage the chunk's node coordinates (precomputed chunk-relative, contiguous -> coalesced),
+//      its incidence list, and the chunk's nodal solution / cotangent / field (node ids sorted)
+//      into LDS;
+//   2. one lane per element: element routine reading LDS; its per-corner results go to an LDS
+//      buffer (no atomics), per-element results straight to HBM;
+//   3. one lane per chunk node sums the node's corner results in a fixed order and writes them
+//      once with a plain store: to the output when the node belongs to this chunk only, to the
+//      node's partial slot when it is shared with other chunks.
+// k_reduce then sums each shared node's partials in chunk order and the per-chunk energies per
+// segment, so the path uses no atomics (device-scope atomics resolve beyond the per-XCD L2) and
+// its results are bitwise reproducible run to run.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -23,51 +25,68 @@
 
 namespace femhot {
 
-// Shared nodes: sum the partials of every chunk containing the node, in chunk order.
+// Both deterministic reductions in one launch.  Blocks [0, nb_nodes) sum, for each shared node,
+// the partials of every chunk containing it (a node's slots are contiguous and in chunk order, so
+// consecutive threads read consecutive memory); the last nseg * batch blocks each sum the
+// per-chunk energies of one (sample, segment) in a fixed order.
+template <class R> struct ReduceArgs {
+  long long n_shared, n_slots;
+  int P, batch, nb_nodes, nseg, n_chunks;
+  const int *sh_nodes, *red_off, *seg_chunk_off;
+  const R *part_u, *part_d;
+  R *ou, *od;
+  long long ous, ods;
+  const uint8_t *dmask;
+  const double *chunk_energy;
+  double *energy;
+  unsigned flags;
+};
+
 template <class R>
-__global__ __launch_bounds__(256) void k_reduce_shared(long long n_shared, int P, int batch, const int *sh_nodes,
-                                                       const int *red_off, const int *red_slots, long long n_slots,
-                                                       const R *part_u, const R *part_d, R *ou, long long ous, R *od,
-                                                       long long ods, const uint8_t *dmask, unsigned flags) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_shared * batch) return;
-  const long long b = i / n_shared;
-  const long long k = i - b * n_shared;
-  const int g = sh_nodes[k];
-  const int j0 = red_off[k], j1 = red_off[k + 1];
-  const bool mask = (flags & FEMHOT_MASK_DIRICHLET) != 0;
-  const bool accumulate = (flags & FEMHOT_ACCUMULATE) != 0;
-  if (ou) {
-    const R *pu = part_u + b * n_slots * P;
+__global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> A) {
+  const bool accumulate = (A.flags & FEMHOT_ACCUMULATE) != 0;
+  if ((int)blockIdx.x >= A.nb_nodes) {
+    __shared__ double red[4];
+    const int q = (int)blockIdx.x - A.nb_nodes;
+    const int b = q / A.nseg, sg = q - b * A.nseg;
+    double v = 0.0;
+    for (int c = A.seg_chunk_off[sg] + (int)threadIdx.x; c < A.seg_chunk_off[sg + 1]; c += 256)
+      v += A.chunk_energy[(long long)b * A.n_chunks + c];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double t = (red[0] + red[1]) + (red[2] + red[3]);
+      double *o = A.energy + (long long)b * A.nseg + sg;
+      *o = accumulate ? *o + t : t;
+    }
+    return;
+  }
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= A.n_shared * A.batch) return;
+  const long long b = i / A.n_shared;
+  const long long k = i - b * A.n_shared;
+  const int g = A.sh_nodes[k];
+  const int j0 = A.red_off[k], j1 = A.red_off[k + 1];
+  const bool mask = (A.flags & FEMHOT_MASK_DIRICHLET) != 0;
+  const int P = A.P;
+  if (A.ou) {
+    const R *pu = A.part_u + b * A.n_slots * P;
     for (int c = 0; c < P; ++c) {
-      R s = R(0);
-      for (int j = j0; j < j1; ++j) s += pu[(long long)red_slots[j] * P + c];
+      R s = R(-0.0);
+      for (int j = j0; j < j1; ++j) s += pu[(long long)j * P + c];
       const long long dof = (long long)g * P + c;
-      const bool masked = mask && dmask[dof];
-      R *o = ou + b * ous + dof;
+      const bool masked = mask && A.dmask[dof];
+      R *o = A.ou + b * A.ous + dof;
       if (!(masked && accumulate)) *o = masked ? R(0) : (accumulate ? *o + s : s);
     }
   }
-  if (od) {
-    const R *pd = part_d + b * n_slots;
-    R s = R(0);
-    for (int j = j0; j < j1; ++j) s += pd[red_slots[j]];
-    R *o = od + b * ods + g;
+  if (A.od) {
+    const R *pd = A.part_d + b * A.n_slots;
+    R s = R(-0.0);
+    for (int j = j0; j < j1; ++j) s += pd[j];
+    R *o = A.od + b * A.ods + g;
     *o = accumulate ? *o + s : s;
-  }
-}
-
-// energy[b, seg] (+)= sum of the per-chunk energies of the segment's chunks, one wave per (b, seg).
-__global__ __launch_bounds__(64) void k_reduce_energy(int nseg, int n_chunks, const int *seg_chunk_off,
-                                                      const double *chunk_energy, double *energy, int accumulate) {
-  const int b = blockIdx.y, s = blockIdx.x;
-  double v = 0.0;
-  for (int c = seg_chunk_off[s] + (int)threadIdx.x; c < seg_chunk_off[s + 1]; c += 64)
-    v += chunk_energy[(long long)b * n_chunks + c];
-  v = wave_sum(v);
-  if (threadIdx.x == 0) {
-    double *o = energy + (long long)b * nseg + s;
-    *o = accumulate ? *o + v : v;
   }
 }
 
@@ -215,25 +234,44 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
     default: rc = launch_chunk_et<TET4>(dtype, c->ph, c->ngp, mode, cfg.block, dim3((unsigned)grid), shm, s, A); break;
   }
   if (rc) return rc;
-  // deterministic reductions of shared nodes and per-chunk energies
-  if (p.n_shared && (ou || want_gd_nodal)) {
-    const long long tot = p.n_shared * batch;
-    const dim3 g((unsigned)((tot + 255) / 256));
-    const long long os = batch > 1 ? ous : c->Ndof, dstr = batch > 1 ? ods : c->Nn;
-    if (dtype == FEMHOT_F32)
-      hipLaunchKernelGGL(k_reduce_shared<float>, g, dim3(256), 0, s, p.n_shared, P, batch, p.sh_nodes, p.red_off,
-                         p.red_slots, p.n_slots, (const float *)part_u, (const float *)part_d, (float *)ou, os,
-                         want_gd_nodal ? (float *)od : nullptr, dstr, c->d_dmask, flags);
-    else
-      hipLaunchKernelGGL(k_reduce_shared<double>, g, dim3(256), 0, s, p.n_shared, P, batch, p.sh_nodes, p.red_off,
-                         p.red_slots, p.n_slots, (const double *)part_u, (const double *)part_d, (double *)ou, os,
-                         want_gd_nodal ? (double *)od : nullptr, dstr, c->d_dmask, flags);
-    FH_HIP(hipGetLastError());
-  }
-  if (energy) {
-    hipLaunchKernelGGL(k_reduce_energy, dim3(c->nseg, batch), dim3(64), 0, s, c->nseg, p.n_chunks, p.seg_chunk_off,
-                       chunk_energy, energy, acc ? 1 : 0);
-    FH_HIP(hipGetLastError());
+  // deterministic reductions of shared nodes and per-chunk energies (one launch)
+  {
+    const bool nodes_part = p.n_shared && (ou || want_gd_nodal);
+    const long long tot = nodes_part ? p.n_shared * batch : 0;
+    const int nb_nodes = (int)((tot + 255) / 256);
+    const int nb_energy = energy ? c->nseg * batch : 0;
+    if (nb_nodes + nb_energy > 0) {
+      const long long os = batch > 1 ? ous : c->Ndof, dstr = batch > 1 ? ods : c->Nn;
+      auto launch = [&](auto zero) -> int {
+        using R = decltype(zero);
+        ReduceArgs<R> ra{};
+        ra.n_shared = nodes_part ? p.n_shared : 0;
+        ra.n_slots = p.n_slots;
+        ra.P = P;
+        ra.batch = batch;
+        ra.nb_nodes = nb_nodes;
+        ra.nseg = c->nseg;
+        ra.n_chunks = p.n_chunks;
+        ra.sh_nodes = p.sh_nodes;
+        ra.red_off = p.red_off;
+        ra.seg_chunk_off = p.seg_chunk_off;
+        ra.part_u = (const R *)part_u;
+        ra.part_d = (const R *)part_d;
+        ra.ou = nodes_part ? (R *)ou : nullptr;
+        ra.od = nodes_part && want_gd_nodal ? (R *)od : nullptr;
+        ra.ous = os;
+        ra.ods = dstr;
+        ra.dmask = c->d_dmask;
+        ra.chunk_energy = chunk_energy;
+        ra.energy = energy;
+        ra.flags = flags;
+        hipLaunchKernelGGL(k_reduce<R>, dim3((unsigned)(nb_nodes + nb_energy)), dim3(256), 0, s, ra);
+        FH_HIP(hipGetLastError());
+        return FEMHOT_OK;
+      };
+      rc = dtype == FEMHOT_F32 ? launch(0.0f) : launch(0.0);
+      if (rc) return rc;
+    }
   }
   return timing_mark(c, s);  // the timed region covers the chunk kernel and its reductions
 }

@@ -166,7 +166,6 @@ static void free_chunks(ChunkPlan &p) {
   (void)hipFree(p.node_slot);
   (void)hipFree(p.sh_nodes);
   (void)hipFree(p.red_off);
-  (void)hipFree(p.red_slots);
   (void)hipFree(p.seg_chunk_off);
   (void)hipFree(p.inc);
   (void)hipFree(p.inc_start);
@@ -290,22 +289,17 @@ int ensure_chunks(Ctx *c, int epc) {
       sh_nodes.push_back((int)g);
     }
   }
-  // partial slots: chunk by chunk, in local-node order; per shared node, its slots in chunk order
+  // partial slots, node-major: the slots of shared node k are red_off[k] .. red_off[k+1] - 1, one per
+  // chunk containing it, in chunk order (so the reduction reads contiguous memory)
   std::vector<int> node_slot(nodes.size(), -1);
   std::vector<int> red_off(sh_nodes.size() + 1, 0);
   for (int g : sh_nodes) red_off[sh_index[g] + 1] = touch[g];
   for (size_t k = 0; k < sh_nodes.size(); ++k) red_off[k + 1] += red_off[k];
-  std::vector<int> red_slots(red_off.back());
   {
     std::vector<int> fillpos(red_off.begin(), red_off.end() - 1);
-    int slot = 0;
     for (size_t k = 0; k < nodes.size(); ++k) {
       const int g = nodes[k];
-      if (touch[g] > 1) {
-        node_slot[k] = slot;
-        red_slots[fillpos[sh_index[g]]++] = slot;
-        ++slot;
-      }
+      if (touch[g] > 1) node_slot[k] = fillpos[sh_index[g]]++;
     }
   }
   std::vector<int> seg_chunk_off(c->nseg + 1, 0);
@@ -318,14 +312,14 @@ int ensure_chunks(Ctx *c, int epc) {
   p.max_elems = max_elems;
   p.n_shared = (long long)sh_nodes.size();
   p.n_zero = (long long)zero.size();
-  p.n_slots = (long long)red_slots.size();
+  p.n_slots = (long long)red_off.back();
   int rc;
   if ((rc = upload(&p.chunk_elem_off, elem_off)) || (rc = upload(&p.chunk_node_off, node_off)) ||
       (rc = upload(&p.chunk_seg, seg)) || (rc = upload(&p.perm, perm)) || (rc = upload(&p.lconn, lconn)) ||
       (rc = upload(&p.chunk_nodes, nodes)) || (rc = upload(&p.chunk_xyz32, xyz32)) ||
       (rc = upload(&p.chunk_xyz64, xyz64)) || (rc = upload(&p.zero_nodes, zero)) ||
       (rc = upload(&p.node_slot, node_slot)) || (rc = upload(&p.sh_nodes, sh_nodes)) ||
-      (rc = upload(&p.red_off, red_off)) || (rc = upload(&p.red_slots, red_slots)) ||
+      (rc = upload(&p.red_off, red_off)) ||
       (rc = upload(&p.seg_chunk_off, seg_chunk_off)) || (rc = upload(&p.inc, inc)) ||
       (rc = upload(&p.inc_start, inc_start))) {
     free_chunks(p);
