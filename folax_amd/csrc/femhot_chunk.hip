@@ -206,6 +206,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   A.batch = batch;
   A.cap = p.max_nodes;
   A.emax = p.max_elems;
+  A.perm_identity = p.perm_identity ? 1 : 0;
   A.n_slots = p.n_slots;
   A.field = field;
   A.fs = batch > 1 ? fs : 0;

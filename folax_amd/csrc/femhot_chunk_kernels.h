@@ -10,6 +10,7 @@ struct ChunkArgs {
   const uint16_t *lconn, *inc, *inc_start;
   const void *xyz;  // chunk-relative coordinates in the arithmetic type
   int n_chunks, n_sblocks, sb, batch, cap, emax;
+  int perm_identity;  // perm[p] == p: the original element id is the position
   long long n_slots;
   const void *field;
   long long fs;
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
         for (int k = 0; k < KE; ++k) {
           const int p = min(base + k * BLOCK + tid, e1 - 1);
           load_lconn<N>(A.lconn + (long long)p * N, l[k]);
-          ee[k] = A.perm[p];
+          ee[k] = A.perm_identity ? p : A.perm[p];
         }
         if (elemental) {
 #pragma unroll

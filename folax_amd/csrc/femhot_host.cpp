@@ -184,6 +184,56 @@ template <class T> static int upload(T **dst, const std::vector<T> &src) {
 // written in contiguous runs).  Each chunk lists its nodes sorted by global id, flags the nodes
 // shared with other chunks, stores its elements' chunk-local connectivity (uint16) and the
 // coordinates of its nodes relative to its first node (FP32 and FP64).
+// Modeled HBM bytes of one chunk-path call for a candidate element order (chunks are runs of at
+// most epc elements within a segment; each run is sorted by element id here, as the plan keeps
+// it): 128-byte lines of the per-element field read and of its gradient written through the
+// order, lines of the nodal solution gathered per chunk, the per-chunk-node plan and output
+// traffic (a shared node adds a partial-slot write and read per chunk), and the order itself
+// unless it is the identity.
+static double chunk_plan_cost(const Ctx *c, std::vector<int> &order, int epc, const std::vector<int> &seg_of,
+                              bool identity) {
+  const int n = c->n;
+  const long long Ne = c->Ne;
+  std::vector<int> loc, touch(c->Nn, 0);
+  double bytes = identity ? 0.0 : 4.0 * (double)Ne;
+  long long chunk_nodes = 0;
+  for (long long i = 0; i < Ne;) {
+    long long j = i;
+    const int s = seg_of[order[i]];
+    while (j < Ne && j - i < epc && seg_of[order[j]] == s) ++j;
+    std::sort(order.begin() + i, order.begin() + j);
+    long long last = -1, le = 0;
+    for (long long k = i; k < j; ++k) {
+      const long long l = (long long)order[k] * 4 / 128;
+      if (l != last) ++le, last = l;
+    }
+    bytes += 2.0 * 128.0 * (double)le;
+    loc.clear();
+    for (long long k = i; k < j; ++k)
+      for (int a = 0; a < n; ++a) loc.push_back(c->h_conn[(size_t)order[k] * n + a]);
+    std::sort(loc.begin(), loc.end());
+    loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+    chunk_nodes += (long long)loc.size();
+    last = -1;
+    long long lu = 0;
+    for (int g : loc) {
+      touch[g]++;
+      const long long l0 = (long long)g * 12 / 128, l1 = ((long long)g * 12 + 11) / 128;
+      if (l0 > last) ++lu, last = l0;
+      if (l1 > last) ++lu, last = l1;
+    }
+    bytes += 128.0 * (double)lu;
+    i = j;
+  }
+  bytes += 26.0 * (double)chunk_nodes;
+  for (long long g = 0; g < c->Nn; ++g)
+    if (touch[g] == 1)
+      bytes += 12.0;
+    else if (touch[g] > 1)
+      bytes += 24.0 * touch[g] + 12.0;
+  return bytes;
+}
+
 int ensure_chunks(Ctx *c, int epc) {
   if (c->chunk.n_chunks && c->chunk.epc == epc) return FEMHOT_OK;
   if (epc < 1 || epc > 8192) {
@@ -225,7 +275,16 @@ int ensure_chunks(Ctx *c, int epc) {
   std::vector<int> cnt;
   std::vector<float> xyz32;
   std::vector<double> xyz64;
-  for (long long i = 0; i < Ne; ++i) perm[i] = key[i].second;
+  // element order: spatial (Morton) chunks, or runs of the caller's own order when that order is
+  // local enough to move fewer bytes (structured and mesher-ordered meshes often are)
+  {
+    std::vector<int> morton(Ne), ident(Ne);
+    for (long long e = 0; e < Ne; ++e) morton[e] = key[e].second, ident[e] = (int)e;
+    const double cost_m = chunk_plan_cost(c, morton, epc, seg_of, false);
+    const double cost_i = chunk_plan_cost(c, ident, epc, seg_of, true);
+    c->chunk.perm_identity = cost_i <= cost_m;
+    perm = c->chunk.perm_identity ? ident : morton;
+  }
   long long i = 0;
   std::vector<int> loc;
   int max_nodes = 0, max_elems = 0;

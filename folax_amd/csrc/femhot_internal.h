@@ -45,6 +45,7 @@ struct ChunkPlan {
   int *chunk_node_off = nullptr;  // (n_chunks+1) offsets into chunk_nodes
   int *chunk_seg = nullptr;       // (n_chunks) segment id
   int *perm = nullptr;            // (Ne) original element id at each position
+  bool perm_identity = false;     // chunks are runs of the caller's element order (perm[i] == i)
   uint16_t *lconn = nullptr;      // (Ne * n) chunk-local node indices, position order
   int *chunk_nodes = nullptr;     // global node id | 0x80000000 when shared
   float *chunk_xyz32 = nullptr;   // per chunk node: (x, y, z, 0) relative to the chunk's first node
