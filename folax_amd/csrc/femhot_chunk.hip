@@ -147,7 +147,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   const bool nodal_gd = od && !elemental && mode != MODE_RES;
   const size_t cap4 = ((size_t)p.max_nodes + 3) & ~(size_t)3;
   const size_t shm = cap4 * (4 + PP + (mode == MODE_VJP ? PP : 0) + (elemental ? 0 : 1)) * rs +
-                     (size_t)p.max_elems * c->n * (PP + (nodal_gd ? 1 : 0)) * rs + (size_t)p.max_elems * c->n * 2 + 16;
+                     (size_t)p.max_elems * c->n * (P + (nodal_gd ? 1 : 0)) * rs + (size_t)p.max_elems * c->n * 2 + 16;
   if (shm > lds_limit()) {
     set_error("run_chunk: chunk too large for LDS; use fewer elements per chunk");
     return FEMHOT_ERR_UNSUPPORTED;
@@ -281,8 +281,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
 // 48 KB, so several workgroups share a CU.
 static ChunkConfig default_config(Ctx *c, int dtype) {
   ChunkConfig cfg;
-  const int PP = c->ndpn == 3 ? 4 : c->ndpn;
-  const size_t per_elem = (size_t)c->n * (PP + 1) * (dtype == FEMHOT_F64 ? 8 : 4);
+  const size_t per_elem = (size_t)c->n * ((c->ndpn + 1) * (dtype == FEMHOT_F64 ? 8 : 4) + 2);
   cfg.epc = 1024;
   while (cfg.epc > 64 && cfg.epc * per_elem > 48 * 1024) cfg.epc /= 2;
   cfg.block = 256;

@@ -60,7 +60,7 @@ template <int P> struct Pad { static constexpr int PP = (P == 3) ? 4 : P; };
 //   U   [cap4 * PP]     nodal solution
 //   W   [cap4 * PP]     cotangent (VJP only)
 //   FD  [cap4]          nodal field (nodal fields only)
-//   ER  [N * emax * PP] per element corner, corner-major ([a][position]): residual / g_u contribution
+//   ER  [P * N * emax]  per element corner, component- and corner-major ([c][a][position]): residual / g_u
 //   ERD [N * emax]      per element corner, corner-major: nodal field-gradient contribution
 //   INC [emax * N] (uint16) the chunk's incidence list
 template <class R, int ET, int PH, int NGP, int MODE, int BLOCK>
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   R *W = U + PP * cap4;
   R *FD = W + (MODE == MODE_VJP ? PP * cap4 : 0);
   R *ER = FD + (elemental ? 0 : cap4);
-  R *ERD = ER + (long)emax * N * PP;
+  R *ERD = ER + (long)emax * N * P;
   uint16_t *INC = reinterpret_cast<uint16_t *>(ERD + (nodal_gd ? (long)emax * N : 0));
   const int tid = threadIdx.x;
   const int n_items = A.n_chunks * A.n_sblocks;
@@ -226,17 +226,11 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             elem_egrad<R, ET, PH, NGP>(el, A.pp, ro, gd, pi);
             en += (double)pi;
           }
-          if (want_ou) {  // corner-major, one 16-byte store per corner when P == 3: consecutive lanes
-                          // write consecutive records
+          if (want_ou) {  // component- and corner-major: consecutive lanes write consecutive words
 #pragma unroll
-            for (int a = 0; a < N; ++a) {
-              if constexpr (P == 3 && sizeof(R) == 4) {
-                reinterpret_cast<float4 *>(ER)[a * emax + pos] = make_float4(ro[a * 3], ro[a * 3 + 1], ro[a * 3 + 2], 0.f);
-              } else {
+            for (int a = 0; a < N; ++a)
 #pragma unroll
-                for (int c = 0; c < P; ++c) ER[(a * emax + pos) * PP + c] = ro[a * P + c];
-              }
-            }
+              for (int c = 0; c < P; ++c) ER[(c * N + a) * emax + pos] = ro[a * P + c];
           }
           if constexpr (MODE != MODE_RES) {
             if (want_gd) {
@@ -286,19 +280,12 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             if (want_ou) {
               R s[P];
 #pragma unroll
-              for (int c = 0; c < P; ++c) s[c] = R(0);
+              for (int c = 0; c < P; ++c) s[c] = R(-0.0);
               for (int j = js[k]; j < je[k]; ++j) {
                 const int q = INC[j];
                 const int r = (q & (N - 1)) * emax + (q >> LOGN);
-                if constexpr (P == 3 && sizeof(R) == 4) {
-                  const float4 v = reinterpret_cast<const float4 *>(ER)[r];
-                  s[0] += v.x;
-                  s[1] += v.y;
-                  s[2] += v.z;
-                } else {
 #pragma unroll
-                  for (int c = 0; c < P; ++c) s[c] += ER[r * PP + c];
-                }
+                for (int c = 0; c < P; ++c) s[c] += ER[c * N * emax + r];
               }
 #pragma unroll
               for (int c = 0; c < P; ++c) {
