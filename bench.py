@@ -246,7 +246,8 @@ def main():
            "warmup": args.warmup, "ms_per_step": round(step_ms, 5), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "32 uneven TET4 elastic meshes (12^3..40^3 hexes x 6 Kuhn tets), residual + "
-                                  "VJP wrt per-element E, one graph-captured persistent launch",
+                                  "VJP wrt per-element E; chunk kernel + one reduction kernel per step, replayed as a "
+                                  "HIP graph",
                       "meshes": nm, "tets": int(Ne), "nodes": int(Nn), "parallelism": "single GPU",
                       "chunk": {"elements_per_chunk": epc, "block": blk, "grid_per_cu": grid},
                       "graph": graph is not None, "graph_replays_match_eager": graph_ok},
