@@ -62,7 +62,8 @@ static int run_elem(Ctx *c, const CallIO &io) {
   const long long fsize = elemental ? c->Ne : c->Nn;
   const bool acc = (io.flags & FEMHOT_ACCUMULATE) != 0;
   int rc;
-  const bool chunk = (io.flags & FEMHOT_SCATTER_CHUNK) != 0;
+  const bool chunk = (io.flags & FEMHOT_SCATTER_CHUNK) != 0 &&
+                     !chunk_prefers_plain(c, io.dtype, io.mode, io.batch, io.flags);
   // clear outputs (the chunk path writes every node itself and clears only shared nodes)
   if (!acc && !chunk) {
     if (io.dtype == FEMHOT_F32) {
@@ -306,10 +307,37 @@ int femhot_autotune(femhot_ctx *h, int op, int dtype, int batch, const void *fie
   io.stream = (hipStream_t)stream;
   int rc = validate(c, io, "femhot_autotune");
   if (rc) return rc;
+  float chunk_ms = 0.f;
   rc = autotune_chunk(c, dtype, op, batch, field, io.batch > 1 ? fs : 0, u, io.batch > 1 ? us : 0, w,
                       io.batch > 1 ? ws : 0, out_u, ous, out_field, ofs, energy, io.flags, io.stream, epc, block,
-                      grid_per_cu, ms);
+                      grid_per_cu, &chunk_ms);
   if (rc) return rc;
+  // the plain path competes too; when it wins, FEMHOT_SCATTER_CHUNK calls of this shape take it
+  CallIO pio = io;
+  pio.flags &= ~FEMHOT_SCATTER_CHUNK;
+  hipEvent_t e0, e1;
+  FH_HIP(hipEventCreate(&e0));
+  FH_HIP(hipEventCreate(&e1));
+  if ((rc = run_elem(c, pio))) return rc;
+  FH_HIP(hipEventRecord(e0, io.stream));
+  constexpr int reps = 3;
+  for (int r = 0; r < reps && !rc; ++r) rc = run_elem(c, pio);
+  FH_HIP(hipEventRecord(e1, io.stream));
+  FH_HIP(hipEventSynchronize(e1));
+  float plain_ms = 0.f;
+  FH_HIP(hipEventElapsedTime(&plain_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  plain_ms /= reps;
+  if (plain_ms < chunk_ms) {
+    record_plain(c, dtype, op, batch, io.flags);
+    if (epc) *epc = 0;
+    if (block) *block = 0;
+    if (grid_per_cu) *grid_per_cu = 0;
+    chunk_ms = plain_ms;
+  }
+  if (ms) *ms = chunk_ms;
   return run_elem(c, io);  // leave valid outputs
 }
 

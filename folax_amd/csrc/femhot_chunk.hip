@@ -103,8 +103,11 @@ __global__ void k_zero_nodes(const int *nodes, long long n, int P, int batch, R 
   if (od) od[b * ods + g] = R(0);
 }
 
+// grid_per_cu 0: one workgroup per work item.  sb: samples per work item (a work item stages its
+// chunk's plan once and loops over sb samples); 0 picks enough items for about 8 per CU.
 struct ChunkConfig {
-  int epc = 1024, block = 256, grid_per_cu = 0, sb = 0;  // grid_per_cu 0: one workgroup per item
+  int epc = 1024, block = 256, grid_per_cu = 0, sb = 0;
+  bool plain = false;  // the plain path won the autotuner for this call shape
 };
 
 static std::mutex g_tune_mu;
@@ -201,7 +204,12 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   A.inc_start = p.inc_start;
   A.xyz = dtype == FEMHOT_F32 ? (const void *)p.chunk_xyz32 : (const void *)p.chunk_xyz64;
   A.n_chunks = p.n_chunks;
-  A.sb = cfg.sb > 0 ? std::min(cfg.sb, batch) : batch;
+  if (cfg.sb > 0) {
+    A.sb = std::min(cfg.sb, batch);
+  } else {  // largest sb that still leaves ~8 work items per CU
+    const long long target = 8LL * cu_count();
+    A.sb = (int)std::max<long long>(1, std::min<long long>(batch, (long long)p.n_chunks * batch / target));
+  }
   A.n_sblocks = (batch + A.sb - 1) / A.sb;
   A.batch = batch;
   A.cap = p.max_nodes;
@@ -322,6 +330,7 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
   static const int epcs[] = {128, 256, 512, 1024, 2048};
   static const int blocks[] = {128, 256, 512};
   static const int grids[] = {0, 4, 8};
+  const int sbs[] = {0, 1};  // automatic, and one sample per work item (only differs when batch > 1)
   hipEvent_t e0, e1;
   FH_HIP(hipEventCreate(&e0));
   FH_HIP(hipEventCreate(&e1));
@@ -330,30 +339,33 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
   for (int epc : epcs) {
     if ((long long)epc > c->Ne * 2 && epc != 128) continue;
     for (int blk : blocks)
-      for (int gp : grids) {
-        ChunkConfig cfg;
-        cfg.epc = epc;
-        cfg.block = blk;
-        cfg.grid_per_cu = gp;
-        int rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, s, cfg);
-        if (rc == FEMHOT_ERR_UNSUPPORTED) continue;  // chunk does not fit LDS
-        if (rc) return rc;
-        FH_HIP(hipEventRecord(e0, s));
-        constexpr int reps = 3;
-        for (int r = 0; r < reps; ++r) {
-          rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, s, cfg);
+      for (int gp : grids)
+        for (int sb : sbs) {
+          if (sb == 1 && batch == 1) continue;
+          ChunkConfig cfg;
+          cfg.epc = epc;
+          cfg.block = blk;
+          cfg.grid_per_cu = gp;
+          cfg.sb = sb;
+          int rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, s, cfg);
+          if (rc == FEMHOT_ERR_UNSUPPORTED) continue;  // chunk does not fit LDS
           if (rc) return rc;
+          FH_HIP(hipEventRecord(e0, s));
+          constexpr int reps = 3;
+          for (int r = 0; r < reps; ++r) {
+            rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, s, cfg);
+            if (rc) return rc;
+          }
+          FH_HIP(hipEventRecord(e1, s));
+          FH_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+          ms /= reps;
+          if (ms < best_t) {
+            best_t = ms;
+            best = cfg;
+          }
         }
-        FH_HIP(hipEventRecord(e1, s));
-        FH_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        FH_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms /= reps;
-        if (ms < best_t) {
-          best_t = ms;
-          best = cfg;
-        }
-      }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -388,6 +400,22 @@ int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int
     g_tuned[{c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)}] = cfg;
   }
   return ensure_chunks(c, epc);
+}
+
+bool chunk_prefers_plain(const Ctx *c, int dtype, int mode, int batch, unsigned flags) {
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find({c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)});
+    if (it != g_tuned.end()) return it->second.plain;
+  }
+  return dtype == FEMHOT_F64 && c->ph == NEOHOOKE;
+}
+
+void record_plain(const Ctx *c, int dtype, int mode, int batch, unsigned flags) {
+  ChunkConfig cfg;
+  cfg.plain = true;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[{c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)}] = cfg;
 }
 
 void forget_tuning(const void *ctx) {

@@ -558,7 +558,7 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
   if ((e = hipMemcpy(c.d_seg_off, c.h_seg_off.data(), sizeof(long long) * (c.nseg + 1), hipMemcpyHostToDevice)) !=
       hipSuccess)
     return fail(e, "hipMemcpy");
-  if ((e = hipHostMalloc((void **)&c.h_pinned, 64 * sizeof(double), hipHostMallocDefault)) != hipSuccess)
+  if ((e = hipHostMalloc((void **)&c.h_pinned, 1024 * sizeof(double), hipHostMallocDefault)) != hipSuccess)
     return fail(e, "hipHostMalloc");
   *out = h;
   return FEMHOT_OK;

@@ -110,7 +110,7 @@ struct Ctx {
   // solver workspace
   size_t ws_bytes = 0;
   void *d_ws = nullptr;
-  double *h_pinned = nullptr;  // small pinned host scalars
+  double *h_pinned = nullptr;  // pinned host scratch (1024 doubles: the solver's reduction partials)
 };
 
 int ensure_pattern(Ctx *c);

@@ -5,40 +5,100 @@
 // BC treatment (Dirichlet rows zeroed except the diagonal, R_D = 0) the Dirichlet unknowns stay
 // zero and the iteration runs on the SPD free block, so CG applies to the matrix as assembled.
 //
-// Design: all CG scalars live in device memory (double, double-buffered by iteration parity), so
-// an iteration is three stream-ordered kernels with no host round trip; the host reads the
-// residual norm only every `check` iterations.  FP32 mode keeps the matrix and vectors in FP32,
-// accumulates every dot product in FP64, and wraps the FP32 solves in FP64 iterative refinement
-// r = b - A x with the FP64 matrix.
+// Design: an iteration is three stream-ordered kernels with no host round trip; the host reads the
+// residual norm only every `check` iterations.  Dot products are reduced without atomics: every
+// reducing kernel runs a fixed grid of NBR workgroups (grid-stride loops) and writes one FP64
+// partial per workgroup; the kernel that needs the scalar sums the NBR partials itself, in a fixed
+// order, in its prologue.  (One device-scope atomic per wave onto a single address serialises: on
+// an 823,875-row system that alone cost ~10 ms per iteration.)  Results are bitwise reproducible.
+// FP32 mode keeps the matrix and vectors in FP32, accumulates every dot product in FP64, and wraps
+// the FP32 solves in FP64 iterative refinement r = b - A x with the FP64 matrix.
 #include <cmath>
 
 #include "femhot_kernels.h"
 
 namespace femhot {
 
-// y = A x with LPR lanes per row; optionally dot += x.y (FP64) into *dot.
+constexpr int NBR = 1024;  // workgroups of every reducing kernel (one FP64 partial each)
+
+// Sum over the workgroup; the result is returned to every thread (blockDim.x == 256).
+__device__ __forceinline__ double block_sum256(double v) {
+  __shared__ double red[4];
+  v = wave_sum(v);
+  __syncthreads();  // red may still be read by a previous call
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Sum of the NBR partials, in a fixed order, returned to every thread.
+__device__ __forceinline__ double sum_partials(const double *part) {
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < NBR / 256; ++k) v += part[k * 256 + threadIdx.x];
+  return block_sum256(v);
+}
+
+// y = A x, LPR lanes per row, one pass (grid = rows * LPR / 256).  Each lane first loads up to U = 8
+// of its row's (column, value) pairs, then gathers x for all of them, so a row costs three dependent
+// memory round trips whatever its length up to 8 * LPR (longer rows continue in a loop).  When part
+// is set, part[block] = this workgroup's share of x.y (FP64).
 template <class T, int LPR>
 __global__ __launch_bounds__(256) void k_spmv(long long nrows, const long long *__restrict__ rp,
                                               const int *__restrict__ ci, const T *__restrict__ v,
-                                              const T *__restrict__ x, T *__restrict__ y, double *dot) {
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long row = tid / LPR;
-  const int lane = (int)(tid % LPR);
-  double s = 0.0;
+                                              const T *__restrict__ x, T *__restrict__ y, double *part) {
+  constexpr int U = 8;
+  const long long row = ((long long)blockIdx.x * 256 + threadIdx.x) / LPR;
+  const int lane = (int)(threadIdx.x % LPR);
+  double d = 0.0;
   if (row < nrows) {
     const long long b = rp[row], e = rp[row + 1];
-    for (long long k = b + lane; k < e; k += LPR) s += (double)v[k] * (double)x[ci[k]];
-  }
+    int cols[U];
+    T vals[U];
 #pragma unroll
-  for (int off = LPR / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-  double d = 0.0;
-  if (row < nrows && lane == 0) {
-    y[row] = (T)s;
-    d = (double)x[row] * (double)(T)s;
+    for (int u = 0; u < U; ++u) {
+      const long long k = b + lane + (long long)u * LPR;
+      const bool in = k < e;
+      cols[u] = in ? ci[k] : (int)row;
+      vals[u] = in ? v[k] : T(0);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += (double)vals[u] * (double)x[cols[u]];
+    for (long long k = b + lane + (long long)U * LPR; k < e; k += LPR) s += (double)v[k] * (double)x[ci[k]];
+#pragma unroll
+    for (int off = LPR / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, LPR);
+    if (lane == 0) {
+      y[row] = (T)s;
+      d = (double)x[row] * (double)(T)s;
+    }
   }
-  if (dot) {
-    d = wave_sum(d);
-    if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(dot, d);
+  if (part) {
+    const double t = block_sum256(d);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
+// out[0] = sum of part[0 .. n) in a fixed order (one workgroup of 1024; four independent
+// accumulators per thread keep four loads in flight).
+__global__ __launch_bounds__(1024) void k_sum_parts(const double *part, long long n, double *out) {
+  __shared__ double red[16];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  long long i = threadIdx.x;
+  for (; i + 3 * 1024 < n; i += 4 * 1024) {
+    a0 += part[i];
+    a1 += part[i + 1024];
+    a2 += part[i + 2 * 1024];
+    a3 += part[i + 3 * 1024];
+  }
+  for (; i < n; i += 1024) a0 += part[i];
+  double v = wave_sum((a0 + a1) + (a2 + a3));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < 16; ++k) t += red[k];
+    out[0] = t;
   }
 }
 
@@ -65,68 +125,52 @@ template <class T> __global__ void k_cast(long long n, const double *a, T *b) {
   for (long long k = i; k < n; k += st) b[k] = (T)a[k];
 }
 
-// r = b - r (r holds A x on entry); p = Dinv r; sc[0] += r.Dinv r ; sc[1] += r.r
+// r = b - r (r holds A x on entry); p = Dinv r; partials of r.Dinv r and r.r
 template <class T>
-__global__ __launch_bounds__(256) void k_init(long long n, const double *b, T *r, T *p, const T *dinv, double *sc) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_init(long long n, const double *b, T *r, T *p, const T *dinv, double *rz_part,
+                                              double *rr_part) {
   double rz = 0.0, rr = 0.0;
-  if (i < n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const double ri = b[i] - (double)r[i];
     const T rt = (T)ri;
     r[i] = rt;
     const T z = rt * dinv[i];
     p[i] = z;
-    rz = (double)rt * (double)z;
-    rr = (double)rt * (double)rt;
+    rz += (double)rt * (double)z;
+    rr += (double)rt * (double)rt;
   }
-  rz = wave_sum(rz);
-  rr = wave_sum(rr);
-  if ((threadIdx.x & 63) == 0) {
-    unsafeAtomicAdd(&sc[0], rz);
-    unsafeAtomicAdd(&sc[1], rr);
-  }
+  rz = block_sum256(rz);
+  rr = block_sum256(rr);
+  if (threadIdx.x == 0) rz_part[blockIdx.x] = rz, rr_part[blockIdx.x] = rr;
 }
 
-// scalars layout: rz[2] at sc[0..1], rr[2] at sc[2..3], pAp[2] at sc[4..5]
+// alpha = (r.z) / (p.Ap); x += alpha p; r -= alpha q; partials of the new r.z and r.r
 template <class T>
-__global__ __launch_bounds__(256) void k_update_xr(long long n, int par, T *x, T *r, const T *p, const T *q,
-                                                   const T *dinv, double *sc) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const double alpha = sc[par] / sc[4 + par];
+__global__ __launch_bounds__(256) void k_update_xr(long long n, T *x, T *r, const T *p, const T *q, const T *dinv,
+                                                   const double *rz_part, const double *pap, double *rz_next,
+                                                   double *rr_next) {
+  const double alpha = sum_partials(rz_part) / pap[0];
   double rz = 0.0, rr = 0.0;
-  if (i < n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     x[i] = (T)((double)x[i] + alpha * (double)p[i]);
     const T ri = (T)((double)r[i] - alpha * (double)q[i]);
     r[i] = ri;
     const T z = ri * dinv[i];
-    rz = (double)ri * (double)z;
-    rr = (double)ri * (double)ri;
+    rz += (double)ri * (double)z;
+    rr += (double)ri * (double)ri;
   }
-  rz = wave_sum(rz);
-  rr = wave_sum(rr);
-  if ((threadIdx.x & 63) == 0) {
-    unsafeAtomicAdd(&sc[1 - par], rz);
-    unsafeAtomicAdd(&sc[2 + 1 - par], rr);
-  }
+  rz = block_sum256(rz);
+  rr = block_sum256(rr);
+  if (threadIdx.x == 0) rz_next[blockIdx.x] = rz, rr_next[blockIdx.x] = rr;
 }
 
+// beta = (r.z)_new / (r.z)_old; p = Dinv r + beta p
 template <class T>
-__global__ __launch_bounds__(256) void k_update_p(long long n, int par, T *p, const T *r, const T *dinv,
-                                                  const double *sc) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const double beta = sc[1 - par] / sc[par];
-  if (i < n) p[i] = (T)((double)(r[i] * dinv[i]) + beta * (double)p[i]);
-}
-
-__global__ void k_zero_next(double *sc, int par) {
-  // zero the accumulators of the next iteration: rz[1-par], rr[1-par], pAp[1-par]
-  sc[1 - par] = 0.0;
-  sc[2 + 1 - par] = 0.0;
-  sc[4 + 1 - par] = 0.0;
-}
-
-__global__ void k_zero_n(double *sc, int n) {
-  if (threadIdx.x < n) sc[threadIdx.x] = 0.0;
+__global__ __launch_bounds__(256) void k_update_p(long long n, T *p, const T *r, const T *dinv, const double *rz_new,
+                                                  const double *rz_old) {
+  const double beta = sum_partials(rz_new) / sum_partials(rz_old);
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    p[i] = (T)((double)(r[i] * dinv[i]) + beta * (double)p[i]);
 }
 
 template <class T> __global__ void k_axpy_to64(long long n, const T *d, double *x) {
@@ -134,43 +178,54 @@ template <class T> __global__ void k_axpy_to64(long long n, const T *d, double *
   if (i < n) x[i] += (double)d[i];
 }
 
+// Lanes per row: the smallest power of two (4..64) whose eight loads per lane cover an average row.
 static int lanes_per_row(long long nnz, long long n) {
   const double avg = n ? (double)nnz / (double)n : 1.0;
   int l = 4;
-  while (l < 64 && l * 2 <= avg) l *= 2;
+  while (l < 64 && 8.0 * l < avg) l *= 2;
   return l;
 }
 
+static unsigned spmv_blocks(long long n, int lpr) { return (unsigned)((n * lpr + 255) / 256); }
+
 template <class T>
-static void spmv_launch(long long n, const long long *rp, const int *ci, const T *v, const T *x, T *y, double *dot,
+static void spmv_launch(long long n, const long long *rp, const int *ci, const T *v, const T *x, T *y, double *part,
                         int lpr, hipStream_t s) {
-  const long long threads = n * lpr;
-  const dim3 grid((unsigned)((threads + 255) / 256));
+  const dim3 grid(spmv_blocks(n, lpr));
   switch (lpr) {
-    case 4: hipLaunchKernelGGL((k_spmv<T, 4>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, dot); break;
-    case 8: hipLaunchKernelGGL((k_spmv<T, 8>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, dot); break;
-    case 16: hipLaunchKernelGGL((k_spmv<T, 16>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, dot); break;
-    case 32: hipLaunchKernelGGL((k_spmv<T, 32>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, dot); break;
-    default: hipLaunchKernelGGL((k_spmv<T, 64>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, dot); break;
+    case 4: hipLaunchKernelGGL((k_spmv<T, 4>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
+    case 8: hipLaunchKernelGGL((k_spmv<T, 8>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
+    case 16: hipLaunchKernelGGL((k_spmv<T, 16>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
+    case 32: hipLaunchKernelGGL((k_spmv<T, 32>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
+    default: hipLaunchKernelGGL((k_spmv<T, 64>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
   }
 }
 
-// Jacobi-PCG in type T on (rp, ci, v) for A x = b (b in FP64), x in T.  Scalars in sc (8 doubles).
+// Host sum (fixed order) of NBR partials copied to pinned memory.
+static int host_sum_partials(Ctx *c, const double *part, hipStream_t s, double *out) {
+  FH_HIP(hipMemcpyAsync(c->h_pinned, part, NBR * sizeof(double), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (int k = 0; k < NBR; ++k) t += c->h_pinned[k];
+  *out = t;
+  return FEMHOT_OK;
+}
+
+// Jacobi-PCG in type T on (rp, ci, v) for A x = b (b in FP64), x in T.  Partials in sc (3 x NBR
+// doubles: r.z for two parities, r.r), pap_part (one per SpMV workgroup) and pap (the summed p.Ap).
 // Returns iterations; *relres = ||r||/||b|| at exit (as tracked by the recurrence).
 template <class T>
 static int pcg_core(Ctx *c, long long n, const long long *rp, const int *ci, const T *v, const double *b, T *x,
-                    T *r, T *p, T *q, T *dinv, double *sc, double bnorm, double tol, int maxiter, int check, int lpr,
-                    hipStream_t s, int *iters_out, double *relres, bool *converged) {
-  const dim3 grid((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL(k_zero_n, dim3(1), dim3(64), 0, s, sc, 8);
+                    T *r, T *p, T *q, T *dinv, double *sc, double *pap_part, double *pap, double bnorm, double tol,
+                    int maxiter, int check, int lpr, hipStream_t s, int *iters_out, double *relres, bool *converged) {
+  const dim3 grid(NBR);
+  double *rz[2] = {sc, sc + NBR}, *rrp = sc + 2 * NBR;
+  const unsigned nsp = spmv_blocks(n, lpr);
   spmv_launch<T>(n, rp, ci, v, x, r, nullptr, lpr, s);  // r = A x
-  hipLaunchKernelGGL((k_init<T>), grid, dim3(256), 0, s, n, b, r, p, dinv, sc);  // rz -> sc[0], rr -> sc[1]
-  // move rr to its slot sc[2]
-  double *hp = c->h_pinned;
-  FH_HIP(hipMemcpyAsync(hp, sc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  FH_HIP(hipStreamSynchronize(s));
-  double rr = hp[1];
-  FH_HIP(hipMemcpyAsync(sc + 2, sc + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL((k_init<T>), grid, dim3(256), 0, s, n, b, r, p, dinv, rz[0], rrp);
+  double rr = 0.0;
+  int rc = host_sum_partials(c, rrp, s, &rr);
+  if (rc) return rc;
   *converged = false;
   *relres = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
   if (bnorm == 0.0 || *relres <= tol) {
@@ -181,15 +236,14 @@ static int pcg_core(Ctx *c, long long n, const long long *rp, const int *ci, con
   int k = 0;
   while (k < maxiter) {
     const int par = k & 1;
-    hipLaunchKernelGGL(k_zero_next, dim3(1), dim3(1), 0, s, sc, par);
-    spmv_launch<T>(n, rp, ci, v, p, q, sc + 4 + par, lpr, s);
-    hipLaunchKernelGGL((k_update_xr<T>), grid, dim3(256), 0, s, n, par, x, r, p, q, dinv, sc);
-    hipLaunchKernelGGL((k_update_p<T>), grid, dim3(256), 0, s, n, par, p, r, dinv, sc);
+    spmv_launch<T>(n, rp, ci, v, p, q, pap_part, lpr, s);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, s, pap_part, (long long)nsp, pap);
+    hipLaunchKernelGGL((k_update_xr<T>), grid, dim3(256), 0, s, n, x, r, p, q, dinv, rz[par], pap, rz[1 - par], rrp);
+    hipLaunchKernelGGL((k_update_p<T>), grid, dim3(256), 0, s, n, p, r, dinv, rz[1 - par], rz[par]);
     ++k;
     if (k % check == 0 || k == maxiter) {
-      FH_HIP(hipMemcpyAsync(hp, sc + 2 + (1 - par), sizeof(double), hipMemcpyDeviceToHost, s));
-      FH_HIP(hipStreamSynchronize(s));
-      rr = hp[0];
+      rc = host_sum_partials(c, rrp, s, &rr);
+      if (rc) return rc;
       *relres = std::sqrt(std::max(rr, 0.0)) / bnorm;
       if (!(std::isfinite(rr))) {
         set_error("femhot_pcg: non-finite residual");
@@ -207,25 +261,26 @@ static int pcg_core(Ctx *c, long long n, const long long *rp, const int *ci, con
   return FEMHOT_OK;
 }
 
-__global__ void k_dot64(long long n, const double *a, const double *b, double *out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  double v = i < n ? a[i] * b[i] : 0.0;
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, v);
+// part[block] = this workgroup's share of a.b
+__global__ __launch_bounds__(256) void k_dot64(long long n, const double *a, const double *b, double *part) {
+  double v = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    v += a[i] * b[i];
+  v = block_sum256(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-// r = b - A x (FP64), returns ||r||^2 into *out (device)
+// r = b - A x (FP64); part[block] = this workgroup's share of ||r||^2
 __global__ __launch_bounds__(256) void k_resid64(long long n, const double *b, const double *ax, double *r,
-                                                 double *out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                 double *part) {
   double v = 0.0;
-  if (i < n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const double ri = b[i] - ax[i];
     r[i] = ri;
-    v = ri * ri;
+    v += ri * ri;
   }
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, v);
+  v = block_sum256(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
 }  // namespace femhot
@@ -255,12 +310,13 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const long long n = c->Ndof, nnz = c->nnz;
-  const int lpr = lanes_per_row(nnz, n);
   const dim3 grid((unsigned)((n + 255) / 256));
   const int check = 10;
-  // workspace: scalars (16 doubles) + 5 FP64 vectors + (FP32 mode) 5 FP32 vectors + FP32 matrix
+  // workspace: reduction partials (5 x NBR doubles) + 5 FP64 vectors + (FP32 mode) 5 FP32 vectors + FP32 matrix
   const size_t vec64 = sizeof(double) * n, vec32 = sizeof(float) * n;
-  size_t need = 256 + 5 * vec64 + 1024;
+  const int lpr = lanes_per_row(nnz, n);
+  const size_t nsp = spmv_blocks(n, lpr);
+  size_t need = (4 * NBR + nsp + 8) * sizeof(double) + 1024 + 5 * vec64 + 1024;
   if (mode == FEMHOT_F32) need += 5 * vec32 + sizeof(float) * nnz + 1024;
   rc = ensure_workspace(c, need);
   if (rc) return rc;
@@ -270,22 +326,25 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
     ws += (bytes + 255) & ~size_t(255);
     return (void *)p;
   };
-  double *sc = (double *)take(16 * sizeof(double));
+  double *sc = (double *)take(3 * NBR * sizeof(double));
+  double *aux = (double *)take(NBR * sizeof(double));
+  double *pap_part = (double *)take(nsp * sizeof(double));
+  double *pap = (double *)take(8 * sizeof(double));
   double *r64 = (double *)take(vec64), *p64 = (double *)take(vec64), *q64 = (double *)take(vec64);
   double *dinv64 = (double *)take(vec64), *ax64 = (double *)take(vec64);
   // ||b||
-  hipLaunchKernelGGL(k_zero_n, dim3(1), dim3(64), 0, s, sc, 16);
-  hipLaunchKernelGGL(k_dot64, grid, dim3(256), 0, s, n, b, b, sc + 8);
-  FH_HIP(hipMemcpyAsync(c->h_pinned, sc + 8, sizeof(double), hipMemcpyDeviceToHost, s));
-  FH_HIP(hipStreamSynchronize(s));
-  const double bnorm = std::sqrt(c->h_pinned[0]);
+  hipLaunchKernelGGL(k_dot64, dim3(NBR), dim3(256), 0, s, n, b, b, aux);
+  double bb = 0.0;
+  rc = host_sum_partials(c, aux, s, &bb);
+  if (rc) return rc;
+  const double bnorm = std::sqrt(bb);
   hipLaunchKernelGGL((k_diag_inv<double>), grid, dim3(256), 0, s, n, row_ptr, col_idx, vals, dinv64);
   int it = 0;
   double rel = 0.0;
   bool conv = false;
   if (mode == FEMHOT_F64) {
-    rc = pcg_core<double>(c, n, row_ptr, col_idx, vals, b, x, r64, p64, q64, dinv64, sc, bnorm, tol, maxiter, check,
-                          lpr, s, &it, &rel, &conv);
+    rc = pcg_core<double>(c, n, row_ptr, col_idx, vals, b, x, r64, p64, q64, dinv64, sc, pap_part, pap, bnorm, tol,
+                          maxiter, check, lpr, s, &it, &rel, &conv);
     if (rc) return rc;
   } else {
     float *v32 = (float *)take(sizeof(float) * nnz);
@@ -298,12 +357,12 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
     const double inner_tol = 1e-4;
     int outer = 0;
     while (true) {
-      hipLaunchKernelGGL(k_zero_n, dim3(1), dim3(64), 0, s, sc + 8, 1);
       spmv_launch<double>(n, row_ptr, col_idx, vals, x, ax64, nullptr, lpr, s);
-      hipLaunchKernelGGL(k_resid64, grid, dim3(256), 0, s, n, b, ax64, r64, sc + 8);
-      FH_HIP(hipMemcpyAsync(c->h_pinned, sc + 8, sizeof(double), hipMemcpyDeviceToHost, s));
-      FH_HIP(hipStreamSynchronize(s));
-      rel = bnorm > 0 ? std::sqrt(c->h_pinned[0]) / bnorm : 0.0;
+      hipLaunchKernelGGL(k_resid64, dim3(NBR), dim3(256), 0, s, n, b, ax64, r64, aux);
+      double r2 = 0.0;
+      rc = host_sum_partials(c, aux, s, &r2);
+      if (rc) return rc;
+      rel = bnorm > 0 ? std::sqrt(r2) / bnorm : 0.0;
       if (!std::isfinite(rel)) {
         set_error("femhot_pcg: non-finite residual");
         return FEMHOT_ERR_NONFINITE;
@@ -318,7 +377,7 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
       double rel_in = 0.0;
       bool cin = false;
       // inner solve on the FP64 residual r64 (FP32 iterates, FP64 dot products)
-      rc = pcg_core<float>(c, n, row_ptr, col_idx, v32, r64, x32, r32, p32, q32, dinv32, sc, bnorm * rel,
+      rc = pcg_core<float>(c, n, row_ptr, col_idx, v32, r64, x32, r32, p32, q32, dinv32, sc, pap_part, pap, bnorm * rel,
                            std::max(inner_tol, 0.5 * tol / std::max(rel, 1e-300)), maxiter - it, check, lpr, s,
                            &k_in, &rel_in, &cin);
       if (rc) return rc;

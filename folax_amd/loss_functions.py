@@ -99,6 +99,12 @@ class FiniteElementLoss:
             raise NotImplementedError("parametric_boundary_learning is not supported by this backend")
         self.loss_function_exponent = float(self.loss_settings.get("loss_function_exponent", 1.0))
         self.control_location = self.loss_settings.get("control_location", "node")
+        # "chunk": LDS-staged, atomic-free and deterministic (the library still takes the plain path
+        # where that is faster: FP64 Neo-Hookean, or where femhot_autotune found it so);
+        # "atomic": one lane per element with hardware atomics
+        self.scatter = self.loss_settings.get("scatter", "chunk")
+        if self.scatter not in ("chunk", "atomic"):
+            raise ValueError(f"scatter must be 'chunk' or 'atomic', got {self.scatter!r}")
         self.ctx = FemhotContext(self.element_type, self.physics, self.fe_mesh.GetNodesCoordinates(),
                                  self.fe_mesh.GetElementsNodes(self.element_type), num_gp=num_gp,
                                  dirichlet_idx=self.dirichlet_indices, dirichlet_val=self.dirichlet_values,
@@ -107,7 +113,8 @@ class FiniteElementLoss:
         self.initialized = True
 
     def _flags(self):
-        return L.ELEMENT_FIELD if self.control_location == "element" else 0
+        f = L.ELEMENT_FIELD if self.control_location == "element" else 0
+        return f | (L.SCATTER_CHUNK if self.scatter == "chunk" else 0)
 
     def _dir_idx(self, device):
         if self._dir_idx_t is None or self._dir_idx_t.device != device:

@@ -143,8 +143,11 @@ int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *c
 
 /* Autotuner for the LDS-staged chunk path (FEMHOT_SCATTER_CHUNK): times elements-per-chunk x
  * workgroup size x grid shape (one workgroup per chunk, or a persistent grid of k workgroups per
- * CU) on these inputs and keeps the fastest for (ctx, op, dtype, batch, field location); then runs
- * the operation once more so the outputs are valid.  Synchronises the stream (not for capture).
+ * CU) x samples per work item on these inputs, and the plain path beside them, and keeps the
+ * fastest for (ctx, op, dtype, batch, field location); FEMHOT_SCATTER_CHUNK calls of that shape then
+ * use it.  *epc = 0 reports that the plain path won.  Untuned, FEMHOT_SCATTER_CHUNK calls take the
+ * chunk path except in FP64 Neo-Hookean, which takes the plain path.  Finally runs the operation
+ * once more so the outputs are valid.  Synchronises the stream (not for capture).
  * op: FEMHOT_OP_RESIDUAL / FEMHOT_OP_VJP / FEMHOT_OP_ENERGY_GRAD.  No counterpart in the
  * reference (XLA fuses the element vmap itself); it replaces fe_loss.py:105-114's fixed batch size. */
 #define FEMHOT_OP_RESIDUAL 0

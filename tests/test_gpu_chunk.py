@@ -37,6 +37,8 @@ def test_chunk_residual_vjp_egrad(kind, etype, dt):
     print(f"[chunk] {kind} {etype} {dt}", flush=True)
     ph = case["ph"]
     ctx = femhot_context(case)
+    for op in (0, 1, 2):  # force the chunk kernel (FP64 Neo-Hookean would otherwise take the plain path)
+        ctx.set_chunk_config(op, TDT[dt], 1, 128, 256, 0)
     f, u, w = dev(case["field"], dt)[None], dev(case["u"], dt)[None], dev(case["w"], dt)[None]
     X = case["coords"][case["conn"]]
     dofs = fe.element_dofs(case["conn"], ph.ndpn)
@@ -124,3 +126,19 @@ def test_chunk_uneven_meshes_element_field(dt):
     assert rel(gd.cpu().numpy(), gde) <= TOL[dt]
     seg_ref = np.array([Pi[eoff[s]:eoff[s + 1]].sum() for s in range(len(parts))])
     assert rel(en.cpu().numpy().ravel(), seg_ref) <= TOL[dt]
+
+
+def test_chunk_flag_takes_plain_path_for_fp64_neohooke():
+    """Untuned FEMHOT_SCATTER_CHUNK calls in FP64 Neo-Hookean take the plain path: same numbers as
+    an explicit plain call, and as the chunk kernel when it is forced."""
+    from tests.cases import femhot_context
+    case = make_case("neohooke", "hexahedron", n=4, seed=5)
+    ctx = femhot_context(case)
+    f, u = dev(case["field"], "f64")[None], dev(case["u"], "f64")[None]
+    en_c, gu_c, gd_c = ctx.energy_grad(f, u, flags=CHUNK)
+    en_p, gu_p, gd_p = ctx.energy_grad(f, u, flags=0)
+    ctx.set_chunk_config(2, torch.float64, 1, 64, 128, 0)
+    en_k, gu_k, gd_k = ctx.energy_grad(f, u, flags=CHUNK)
+    torch.cuda.synchronize()
+    for a, b in ((gu_c, gu_p), (gd_c, gd_p), (en_c, en_p), (gu_k, gu_p), (gd_k, gd_p), (en_k, en_p)):
+        assert rel(a.cpu().numpy(), b.cpu().numpy()) <= 1e-12
