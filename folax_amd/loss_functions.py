@@ -105,12 +105,25 @@ class FiniteElementLoss:
         self.scatter = self.loss_settings.get("scatter", "chunk")
         if self.scatter not in ("chunk", "atomic"):
             raise ValueError(f"scatter must be 'chunk' or 'atomic', got {self.scatter!r}")
+        # "autotune": time chunk configurations (and the plain path) the first time each call shape
+        # (operation, dtype, batch) is seen, and keep the fastest (femhot_autotune)
+        self.autotune = bool(self.loss_settings.get("autotune", False))
+        self._tuned = set()
         self.ctx = FemhotContext(self.element_type, self.physics, self.fe_mesh.GetNodesCoordinates(),
                                  self.fe_mesh.GetElementsNodes(self.element_type), num_gp=num_gp,
                                  dirichlet_idx=self.dirichlet_indices, dirichlet_val=self.dirichlet_values,
                                  segment_offsets=self.loss_settings.get("segment_offsets"), **self._material())
         self._dir_idx_t = None
         self.initialized = True
+
+    def _tune(self, op, params, dofs, w=None):
+        """First call of a shape with autotune on: femhot_autotune picks the fastest configuration."""
+        if not (self.autotune and self.scatter == "chunk"):
+            return
+        key = (op, dofs.dtype, dofs.shape[0] if dofs.dim() > 1 else 1)
+        if key not in self._tuned:
+            self.ctx.autotune(op, params, dofs, w, flags=self._flags() & ~L.SCATTER_CHUNK)
+            self._tuned.add(key)
 
     def _flags(self):
         f = L.ELEMENT_FIELD if self.control_location == "element" else 0
@@ -245,6 +258,7 @@ class _BatchEnergy(torch.autograd.Function):
     def forward(ctx, loss, params, dofs):
         params = params.to(dofs.dtype).contiguous()
         dofs = dofs.contiguous()
+        loss._tune(L.OP_RESIDUAL, params, dofs)
         _, en = loss.ctx.residual(params, dofs, want_R=False, flags=loss._flags())
         ctx.loss = loss
         ctx.save_for_backward(params, dofs)
@@ -256,6 +270,7 @@ class _BatchEnergy(torch.autograd.Function):
         params, dofs = ctx.saved_tensors
         flags = loss._flags()
         if loss.physics == L.NEOHOOKE:
+            loss._tune(L.OP_ENERGY_GRAD, params, dofs)
             _, gu, gd = loss.ctx.energy_grad(params, dofs, want_energy=False, flags=flags)
             if np.any(np.asarray(loss.loss_settings.get("body_foce", [0.0]), dtype=np.float64) != 0.0):
                 # Ee excludes the body-force work: add Fe = -r(u = 0) back
@@ -274,6 +289,7 @@ class _ResidualNormLoss(torch.autograd.Function):
         params = params.to(dofs.dtype).contiguous()
         dofs = dofs.contiguous()
         flags = loss._flags()
+        loss._tune(L.OP_RESIDUAL, params, dofs)
         R, _ = loss.ctx.residual(params, dofs, want_energy=False, flags=flags | L.MASK_DIRICHLET)
         ctx.loss, ctx.picard = loss, picard
         ctx.save_for_backward(params, dofs, R)
@@ -284,6 +300,7 @@ class _ResidualNormLoss(torch.autograd.Function):
         loss = ctx.loss
         params, dofs, R = ctx.saved_tensors
         flags = loss._flags() | (L.PICARD if ctx.picard else 0)
+        loss._tune(L.OP_VJP, params, dofs, R)
         gu, gd = loss.ctx.residual_vjp(params, dofs, R, flags=flags | L.MASK_DIRICHLET)
         s = g.to(dofs.dtype)
         return None, gd * s, gu * s, None

@@ -180,15 +180,17 @@ def test_batch_loss_value_and_autograd(kind, exponent):
             assert float(Pt.grad[b].abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("autotune", [False, True])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_residual_norm_loss_and_vjp(dtype):
-    """1/2 sum_b ||R_free||^2 for nonlinear thermal QUAD4 and its exact gradient (w.r.t. T and k0)."""
+def test_residual_norm_loss_and_vjp(dtype, autotune):
+    """1/2 sum_b ||R_free||^2 for nonlinear thermal QUAD4 and its exact gradient (w.r.t. T and k0),
+    with the library's default configuration and after the loss autotunes its call shapes."""
     from folax_amd import loss_functions as lf
     from folax_amd.mesh import create_2D_square_mesh
     m = create_2D_square_mesh(1.0, 17)
     coords, conn = m.GetNodesCoordinates(), m.GetElementsNodes("quad")
     bc = {"T": {"left": 1.0, "right": 0.0}}
-    loss = lf.ThermalLoss2DQuad("t", {"dirichlet_bc_dict": bc, "beta": 0.5, "c": 2}, m)
+    loss = lf.ThermalLoss2DQuad("t", {"dirichlet_bc_dict": bc, "beta": 0.5, "c": 2, "autotune": autotune}, m)
     loss.Initialize()
     rng = np.random.default_rng(8)
     B = 4
