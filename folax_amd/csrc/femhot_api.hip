@@ -255,14 +255,14 @@ int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *
       if (!cnt) continue;
       A.list = c->d_colour_elems + c->h_colour_off[k];
       A.count = cnt;
-      L.grid = dim3((unsigned)((cnt * n + 255) / 256));
+      L.grid = dim3((unsigned)((cnt * n * c->ndpn + 255) / 256));
       if ((rc = launch_tangent_any(c->et, L))) return rc;
     }
   } else {
     L.plain = false;
     A.list = nullptr;
     A.count = c->Ne;
-    L.grid = dim3((unsigned)((c->Ne * n + 255) / 256));
+    L.grid = dim3((unsigned)((c->Ne * n * c->ndpn + 255) / 256));
     if ((rc = launch_tangent_any(c->et, L))) return rc;
   }
   if (R) {

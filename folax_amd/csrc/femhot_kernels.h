@@ -230,17 +230,18 @@ struct TArgs {
   PhysParams pp;
 };
 
-// One lane per (element, local node a): rows (a, c) of the element tangent, BC-masked as
-// fe_loss.py:191-207 (Dirichlet rows keep only the element diagonal), added into the CSR values
-// at row_ptr[row] + P*slot(e,a,b) + d.
+// One lane per (element, local node a, component c): row (a, c) of the element tangent, BC-masked as
+// fe_loss.py:191-207 (Dirichlet rows keep only the element diagonal), added into the CSR values at
+// row_ptr[row] + P*slot(e,a,b) + d.
 template <class R, int ET, int PH, int NGP, bool PLAIN>
 __global__ __launch_bounds__(256) void k_tangent(TArgs A) {
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, P = E::P, NL = E::NL;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.count * N) return;
-  const long long ei = i / N;
-  const int a = (int)(i - ei * N);
+  if (i >= A.count * N * P) return;
+  const long long ei = i / (N * P);
+  const int ac = (int)(i - ei * (N * P));
+  const int a = ac / P, c = ac - a * P;
   const long long e = A.list ? (long long)A.list[ei] : ei;
   int nodes[N];
   load_conn<N>(A.conn, e, nodes);
@@ -258,39 +259,39 @@ __global__ __launch_bounds__(256) void k_tangent(TArgs A) {
 #pragma unroll
     for (int b = 0; b < N; ++b) el.d[b] = F[nodes[b]];
   }
+  if constexpr (PH != MECH) {  // the linear-elastic tangent does not depend on u
 #pragma unroll
-  for (int b = 0; b < N; ++b)
+    for (int b = 0; b < N; ++b)
 #pragma unroll
-    for (int c = 0; c < P; ++c) el.u[b * P + c] = U[(size_t)nodes[b] * P + c];
-  R K[P][NL];
-  elem_tangent_rows<R, ET, PH, NGP>(el, A.pp, a, K);
-  int na = 0;
+      for (int d = 0; d < P; ++d) el.u[b * P + d] = U[(size_t)nodes[b] * P + d];
+  }
+  R K[NL];
+  elem_tangent_row<R, ET, PH, NGP>(el, A.pp, a, c, K);
+  int na = nodes[0];
 #pragma unroll
-  for (int b = 0; b < N; ++b) na = (b == a) ? nodes[b] : na;
+  for (int b = 1; b < N; ++b) na = (b == a) ? nodes[b] : na;
   const uint8_t *sl = A.slot + ((size_t)e * N + a) * N;
   uint8_t ks[N];
 #pragma unroll
   for (int b = 0; b < N; ++b) ks[b] = sl[b];
   R *V = reinterpret_cast<R *>(A.vals);
+  const long long row = (long long)na * P + c;
+  const long long rp = A.row_ptr[row];
+  if (A.dmask[row]) {
+    R kd = R(0);
+    int kaa = 0;
 #pragma unroll
-  for (int c = 0; c < P; ++c) {
-    const long long row = (long long)na * P + c;
-    const long long rp = A.row_ptr[row];
-    if (A.dmask[row]) {
-      R kd = R(0);
-      int kaa = 0;
+    for (int b = 0; b < N; ++b) {
 #pragma unroll
-      for (int b = 0; b < N; ++b) {
-        kd = (b == a) ? K[c][b * P + c] : kd;
-        kaa = (b == a) ? ks[b] : kaa;
-      }
-      add_out<R>(V + rp + (long long)P * kaa + c, kd, PLAIN);
-    } else {
-#pragma unroll
-      for (int b = 0; b < N; ++b)
-#pragma unroll
-        for (int d = 0; d < P; ++d) add_out<R>(V + rp + (long long)P * ks[b] + d, K[c][b * P + d], PLAIN);
+      for (int d = 0; d < P; ++d) kd = (b == a && d == c) ? K[b * P + d] : kd;  // selects: no dynamic index
+      kaa = (b == a) ? ks[b] : kaa;
     }
+    add_out<R>(V + rp + (long long)P * kaa + c, kd, PLAIN);
+  } else {
+#pragma unroll
+    for (int b = 0; b < N; ++b)
+#pragma unroll
+      for (int d = 0; d < P; ++d) add_out<R>(V + rp + (long long)P * ks[b] + d, K[b * P + d], PLAIN);
   }
 }
 

@@ -192,6 +192,8 @@ class FiniteElementLoss:
         u = _as_device_tensor(total_primal_vars)
         d = _as_device_tensor(total_control_vars, dtype=u.dtype)
         flags = self._flags() | (L.TRANSPOSE if transpose_jacobian else 0)
+        if self.scatter == "chunk":  # colour-ordered assembly: deterministic, and faster than atomics
+            flags |= L.SCATTER_COLOUR
         vals, R = self.ctx.tangent_csr(d, u, want_R=True, flags=flags)
         if not hasattr(self, "_pattern") or self._pattern[0].device != u.device:
             self._pattern = self.ctx.pattern(u.device)
