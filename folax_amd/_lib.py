@@ -68,6 +68,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `python -c 'import folax_amd.build as b; b.build()'`")
+    # torch's ROCm wheel carries its own HIP runtime (file libamdhip64.so, SONAME libamdhip64.so.7).
+    # Loading torch first makes the library's libamdhip64.so.7 resolve to that same copy; loading the
+    # library first would bring in /opt/rocm's runtime and leave two HIP runtimes in the process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, ll, i32, u32, dbl = C.c_void_p, C.c_longlong, C.c_int, C.c_uint, C.c_double
     L.femhot_last_error.restype = C.c_char_p
