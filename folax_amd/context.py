@@ -182,6 +182,19 @@ class FemhotContext:
                                              _ptr(gu), self.n_dofs, _ptr(gd), fsz, flags, _stream()))
         return (en.view(B, self.n_segments) if en is not None else None), gu, gd
 
+    def residual_shape_vjp(self, field, u, w, flags=0):
+        """(dR/dX)^T w as an (n_nodes, 3) tensor (one sample): the shape sensitivity of the residual."""
+        u, w = u.reshape(-1), w.reshape(-1)
+        field = field.reshape(-1)
+        dt, dev = u.dtype, u.device
+        self._check_in(u.view(1, -1), "u", self.n_dofs, 1, dt, dev)
+        self._check_in(w.view(1, -1), "w", self.n_dofs, 1, dt, dev)
+        self._check_in(field.view(1, -1), "field", self._field_size(flags), 1, dt, dev)
+        gx = torch.empty(self.n_nodes, 3, dtype=dt, device=dev)
+        L.check(self._lib.femhot_residual_shape_vjp(self._h, _dtype_code(u), _ptr(field), _ptr(u), _ptr(w), _ptr(gx),
+                                                   flags, _stream()))
+        return gx
+
     def set_chunk_config(self, op, dtype, batch, epc, block, grid_per_cu=0, flags=0):
         code = L.F32 if dtype == torch.float32 else L.F64
         L.check(self._lib.femhot_set_chunk_config(self._h, int(op), code, int(batch), int(flags), int(epc),

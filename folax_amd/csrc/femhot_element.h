@@ -207,6 +207,9 @@ __device__ __forceinline__ void gp_geometry(int g, const R (&x)[ElemT<ET>::N][El
   G.wdet = R(w) * det;
 }
 
+// Base arithmetic type of R (R itself; Dual<T> maps to T): series lengths follow the base precision.
+template <class R> struct BaseType { using T = R; };
+
 // ---------------------------------------------------------------------------------------------
 // small math helpers
 // ---------------------------------------------------------------------------------------------
@@ -242,7 +245,7 @@ template <class R> __device__ __forceinline__ R xm_log1p(R x) {
   if (fabs(x) < R(0.5)) {
     const R s = x / (R(2) + x);
     const R s2 = s * s;
-    constexpr int K = sizeof(R) == 8 ? 16 : 8;
+    constexpr int K = sizeof(typename BaseType<R>::T) == 8 ? 16 : 8;
     R t = R(-0.0);
 #pragma unroll
     for (int k = K; k >= 1; --k) t = t * s2 + R(1) / R(2 * k + 1);
@@ -254,7 +257,7 @@ template <class R> __device__ __forceinline__ R xm_log1p(R x) {
 // expm1(x) - x without cancellation (Taylor series for |x| < 0.5).
 template <class R> __device__ __forceinline__ R expm1_mx(R x) {
   if (fabs(x) < R(0.5)) {
-    constexpr int K = sizeof(R) == 8 ? 17 : 10;
+    constexpr int K = sizeof(typename BaseType<R>::T) == 8 ? 17 : 10;
     R t = R(-0.0);
     // sum_{k>=2} x^k / k!  = x^2 (1/2! + x/3! + ...)
 #pragma unroll

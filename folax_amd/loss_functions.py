@@ -218,6 +218,15 @@ class FiniteElementLoss:
         gu, gd = self.ctx.residual_vjp(d.reshape(1, -1), u.reshape(1, -1), w.reshape(1, -1), flags=f)
         return gu[0], gd[0]
 
+    def ComputeResidualShapeVJP(self, total_control_vars, total_primal_vars, cotangent):
+        """(dR/dX)^T w, the adjoint shape-derivative term (fe_response.py:312-332, 365-406), flattened
+        as (3 * n_nodes,) like the reference's nodal shape derivative vector."""
+        u = _as_device_tensor(total_primal_vars)
+        d = _as_device_tensor(total_control_vars, dtype=u.dtype)
+        w = _as_device_tensor(cotangent, dtype=u.dtype)
+        f = L.ELEMENT_FIELD if self.control_location == "element" else 0
+        return self.ctx.residual_shape_vjp(d, u, w, flags=f).reshape(-1)
+
     def ComputePotentialAndGradient(self, total_control_vars, total_primal_vars):
         """Potential Pi with dPi/du (= residual) and dPi/dcontrol."""
         u = _as_device_tensor(total_primal_vars)

@@ -21,6 +21,8 @@
  *                                           -> femhot_residual / femhot_tangent_csr
  *  - fol/responses/fe_response.py:434-526   ComputeLossElementControlGrad / adjoint nodal control
  *                                           derivatives (lambda^T dR/dk) -> femhot_residual_vjp
+ *  - fol/responses/fe_response.py:312-332,365-406 adjoint nodal shape derivatives (lambda^T dR/dX)
+ *                                           -> femhot_residual_shape_vjp
  *  - jax.grad of fe_loss.py:175 ComputeTotalEnergy (VJP of the energy) -> femhot_energy_grad
  *  - fol/solvers/fe_solver.py:60-78         LinearSolve (BiCGSTAB / direct on the assembled
  *                                           tangent) -> femhot_pcg (Jacobi-PCG, FP64 or FP32
@@ -126,6 +128,14 @@ int femhot_residual_vjp(femhot_ctx *ctx, int dtype, int batch, const void *field
                         const void *u, long long u_stride, const void *w, long long w_stride, void *g_u,
                         long long g_u_stride, void *g_field, long long g_field_stride, unsigned flags,
                         void *stream);
+
+/* VJP of the residual with respect to the node coordinates (shape sensitivity): g_x = (dR/dX)^T w,
+ * (n_nodes, 3) in the arithmetic type, overwritten; the z column is zero for 2-D elements.  Replaces
+ * jax.jacrev(ComputeElement[1], argnums=0) contracted with the element adjoint and assembled per
+ * node component (fol/responses/fe_response.py:312-332, 365-406).  One sample; FEMHOT_ELEMENT_FIELD
+ * is honoured, other flags are ignored. */
+int femhot_residual_shape_vjp(femhot_ctx *ctx, int dtype, const void *field, const void *u, const void *w,
+                              void *g_x, unsigned flags, void *stream);
 
 /* Potential Pi and its gradient: g_u = dPi/du (= residual), g_field = dPi/dfield.
  * energy: double (batch * max(1,n_segments)) or NULL. */

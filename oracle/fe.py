@@ -125,7 +125,8 @@ def geometry(etype: str, X, num_gp: int):
     """
     pts, wts = gauss_rule(etype, num_gp)
     dim = ELEM_DIM[etype]
-    Xd = np.asarray(X, dtype=np.float64)[..., :dim]
+    X = np.asarray(X)
+    Xd = (X if np.iscomplexobj(X) else X.astype(np.float64))[..., :dim]  # complex kept for complex-step
     N = np.stack([shape_values(etype, p) for p in pts])
     dN = np.stack([shape_local_grads(etype, p) for p in pts])
     J = np.einsum("eai,gak->egik", Xd, dN)
@@ -200,13 +201,13 @@ def b_matrix(DN):
     n, dim = DN.shape[-2], DN.shape[-1]
     lead = DN.shape[:-2]
     if dim == 2:
-        B = np.zeros(lead + (3, 2 * n))
+        B = np.zeros(lead + (3, 2 * n), dtype=DN.dtype)
         B[..., 0, 0::2] = DN[..., 0]
         B[..., 1, 1::2] = DN[..., 1]
         B[..., 2, 0::2] = DN[..., 1]
         B[..., 2, 1::2] = DN[..., 0]
         return B
-    B = np.zeros(lead + (6, 3 * n))
+    B = np.zeros(lead + (6, 3 * n), dtype=DN.dtype)
     B[..., 0, 0::3] = DN[..., 0]
     B[..., 1, 1::3] = DN[..., 1]
     B[..., 2, 2::3] = DN[..., 2]
@@ -656,6 +657,45 @@ def global_residual_vjp(ph, coords, conn, field, u, w, picard=False):
     else:
         g_d = gd_e
     return g_u, g_d
+
+
+def global_residual_shape_vjp(ph, coords, conn, field, u, w, h=None):
+    """(dR/dX)^T w assembled per node component, (Nn, 3): the adjoint shape-derivative term of
+    fol/responses/fe_response.py:312-332 (jacrev of ComputeElement[1] w.r.t. the element coordinates,
+    contracted with the element adjoint) and :365-406 (assembly).
+
+    Thermal and linear-elastic elements are differentiated by complex step (exact to rounding: the
+    geometry keeps complex coordinates).  The Neo-Hookean oracle evaluates psi/S in long double, which
+    has no complex form, so it takes central differences (step h, accuracy ~1e-9 relative)."""
+    coords = np.asarray(coords, dtype=np.float64)
+    conn = np.asarray(conn)
+    field = np.asarray(field, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    w = np.asarray(w, dtype=np.float64)
+    dofs = element_dofs(conn, ph.ndpn)
+    X = coords[conn]
+    de = field[conn] if field.shape[0] == coords.shape[0] and field.ndim == 1 and field.shape[0] != conn.shape[0] \
+        else field
+    ue, we = u[dofs], w[dofs]
+    n = conn.shape[1]
+    dim = ELEM_DIM[ph.etype]
+    gx = np.zeros((coords.shape[0], 3))
+    for a in range(n):
+        for i in range(dim):
+            if ph.kind != "neohooke":
+                hc = 1e-30
+                Xc = X.astype(complex)
+                Xc[:, a, i] += 1j * hc
+                _, r, _ = ph.element(Xc, de, ue, want_K=False)
+                dr = np.imag(r) / hc
+            else:
+                hh = h if h is not None else 1e-6 * max(1.0, float(np.abs(X).max()))
+                Xp, Xm = X.copy(), X.copy()
+                Xp[:, a, i] += hh
+                Xm[:, a, i] -= hh
+                dr = (ph.element(Xp, de, ue, want_K=False)[1] - ph.element(Xm, de, ue, want_K=False)[1]) / (2 * hh)
+            np.add.at(gx[:, i], conn[:, a], np.einsum("ek,ek->e", np.real(dr), we))
+    return gx
 
 
 def global_energy_grad(ph, coords, conn, field, u):

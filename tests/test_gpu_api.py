@@ -65,6 +65,11 @@ def test_sensitivity_analysis_golden_through_api():
     _, dfdE = _response_grads(m.GetNodesCoordinates(), m.GetElementsNodes("quad"), K, u)
     chk = sa["checks"][2]
     np.testing.assert_allclose(dfdE + gd.cpu().numpy(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+    from tests.test_oracle_goldens import _response_shape_grad
+    gx = loss.ComputeResidualShapeVJP(t64(K), t64(u), t64(lam)).cpu().numpy()
+    chk = sa["checks"][3]
+    dfdX = _response_shape_grad(m.GetNodesCoordinates(), m.GetElementsNodes("quad"), K, u).ravel()
+    np.testing.assert_allclose(dfdX + gx, chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
 
 
 @pytest.mark.parametrize("solver", ["PCG", "PCG-mixed"])
@@ -87,6 +92,11 @@ def test_mechanical_2D_sa_golden_through_api(solver):
     _, gd = loss.ComputeResidualVJP(t64(K), u, lam)
     chk = sa["test_sensitivites"]["checks"][0]
     np.testing.assert_allclose(dfdE + gd.cpu().numpy(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+    from tests.test_oracle_goldens import _response_shape_grad
+    gx = loss.ComputeResidualShapeVJP(t64(K), u, lam).cpu().numpy()
+    chk = sa["test_sensitivites"]["checks"][1]
+    dfdX = _response_shape_grad(coords, conn, K, u.cpu().numpy()).ravel()
+    np.testing.assert_allclose(dfdX + gx, chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
 
 
 def _c1_inputs():

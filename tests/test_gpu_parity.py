@@ -205,3 +205,24 @@ def test_pcg_against_direct_solve(kind, etype, mode):
     torch.cuda.synchronize()
     assert ok and relres <= 1e-11
     assert rel(x.cpu().numpy(), du_ref) <= 1e-9
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("kind,etype", ALL)
+def test_residual_shape_vjp(kind, etype, dt):
+    """(dR/dX)^T w on the GPU (forward-mode duals) against the oracle (complex step; central
+    differences for Neo-Hookean), and translation invariance of the result."""
+    from tests.cases import femhot_context
+    case = make_case(kind, etype, n=3, seed=51, **(dict(beta=0.5, c=2.0) if kind == "thermal" else {}))
+    print(f"[shape] {kind} {etype} {dt}", flush=True)
+    ph = case["ph"]
+    ref = fe.global_residual_shape_vjp(ph, case["coords"], case["conn"], case["field"], case["u"], case["w"])
+    ctx = femhot_context(case)
+    gx = ctx.residual_shape_vjp(dev(case["field"], dt), dev(case["u"], dt), dev(case["w"], dt))
+    torch.cuda.synchronize()
+    gx = gx.cpu().numpy()
+    tol = {"f64": 1e-7 if kind == "neohooke" else 1e-12, "f32": 1e-5}[dt]
+    assert rel(gx, ref) <= tol
+    assert np.abs(gx.sum(axis=0)).max() <= (1e-12 if dt == "f64" else 1e-5) * np.abs(gx).max() * gx.shape[0]
+    if ph.dim == 2:
+        assert np.all(gx[:, 2] == 0.0)

@@ -113,6 +113,20 @@ def _response_grads(coords, conn, K, u):
     return dfdu, dfdE
 
 
+def _response_shape_grad(coords, conn, K, u):
+    """Shape gradient of the same response: only detJ depends on the coordinates, and
+    d detJ / dX_(a,i) = detJ dN_a/dx_i (Jacobi's formula), so df/dX_(a,i) = sum_g w f_g detJ DN_(a,i).
+    (Nn, 3) with a zero z column (2-D element)."""
+    N, w, DN, detJ = fe.geometry("quad", coords[conn], 2)
+    Eg = np.einsum("ga,ea->eg", N, K[conn])
+    U0 = np.einsum("ga,ea->eg", N, u[fe.element_dofs(conn, 2)][:, 0::2])
+    fw = w[None] * Eg ** 2 * U0 * detJ
+    g = np.zeros((coords.shape[0], 3))
+    for i in range(2):
+        np.add.at(g[:, i], conn, np.einsum("eg,ega->ea", fw, DN[..., i]))
+    return g
+
+
 def test_sensitivity_analysis_goldens():
     """tests/unit/test_sensitivity_analysis.py: transposed BC tangent row 8, adjoint RHS and
     control derivatives lambda^T dR/dE + df/dE on a 3x3-node QUAD4 mesh."""
@@ -128,6 +142,9 @@ def test_sensitivity_analysis_goldens():
     _check(-rhs, SA["checks"][1])
     _, gd = fe.global_residual_vjp(ph, coords, conn, K, u, lam)
     _check(dfdE + gd, SA["checks"][2])
+    gx = fe.global_residual_shape_vjp(ph, coords, conn, K, u, lam)
+    assert SA["checks"][3]["actual"].startswith("shape_derivatives")
+    _check((_response_shape_grad(coords, conn, K, u) + gx).ravel(), SA["checks"][3])
 
 
 def test_mechanical_2D_sa_end_to_end_golden():
@@ -146,3 +163,7 @@ def test_mechanical_2D_sa_end_to_end_golden():
     chk = SA2["test_sensitivites"]["checks"][0]
     assert chk["actual"] == "control_derivatives"
     _check(dfdE + gd, chk)
+    chk = SA2["test_sensitivites"]["checks"][1]
+    assert chk["actual"] == "shape_derivatives"
+    gx = fe.global_residual_shape_vjp(ph, coords, conn, K, u, lam)
+    _check((_response_shape_grad(coords, conn, K, u) + gx).ravel(), chk)

@@ -275,3 +275,54 @@ def test_c_restatement_matches_numpy_oracle():
     np.testing.assert_allclose(R, fe.scatter_nodal(conn, r, coords.shape[0], 3), rtol=1e-12, atol=1e-18)
     np.testing.assert_allclose(gE, gde, rtol=1e-12)
     np.testing.assert_allclose(Pi, [Pe[seg == 0].sum(), Pe[seg == 1].sum()], rtol=1e-12)
+
+
+def _wR_of_coords(case, coords):
+    ph = case["ph"]
+    dofs = fe.element_dofs(case["conn"], ph.ndpn)
+    _, r, _ = ph.element(coords[case["conn"]], case["field"][case["conn"]], case["u"][dofs], want_K=False)
+    return float(np.einsum("ek,ek->", r, case["w"][dofs]))
+
+
+@pytest.mark.parametrize("kind,etype", ALL)
+def test_shape_vjp_finite_differences_and_translation(kind, etype):
+    """(dR/dX)^T w against central differences of w.R(X) along random coordinate directions, and
+    translation invariance (R does not change when every node moves by the same vector, so the
+    shape gradient sums to zero per component; 2-D elements have a zero z column)."""
+    case = make_case(kind, etype, n=2, seed=41, **(dict(beta=0.5, c=2.0) if kind == "thermal" else {}))
+    ph = case["ph"]
+    gx = fe.global_residual_shape_vjp(ph, case["coords"], case["conn"], case["field"], case["u"], case["w"])
+    rng = np.random.default_rng(42)
+    for _ in range(3):
+        v = rng.standard_normal(case["coords"].shape)
+        if ph.dim == 2:
+            v[:, 2] = 0.0
+        eps = 1e-6
+        fd = (_wR_of_coords(case, case["coords"] + eps * v) - _wR_of_coords(case, case["coords"] - eps * v)) / (2 * eps)
+        assert abs(fd - np.sum(gx * v)) <= 1e-6 * max(abs(fd), np.abs(gx).sum() * 1e-3)
+    # complex step is exact to rounding; the Neo-Hookean oracle takes central differences
+    assert np.abs(gx.sum(axis=0)).max() <= (1e-8 if kind == "neohooke" else 1e-12) * np.abs(gx).max()
+    if ph.dim == 2:
+        assert np.all(gx[:, 2] == 0.0)
+
+
+def test_planted_shape_vjp_without_detj_term_is_caught():
+    """A shape gradient that ignores how detJ moves with the nodes fails the difference pin."""
+    case = make_case("mechanical", "quad", n=2, seed=43)
+    ph = case["ph"]
+    gx = fe.global_residual_shape_vjp(ph, case["coords"], case["conn"], case["field"], case["u"], case["w"])
+    N, w, DN, detJ = fe.geometry("quad", case["coords"][case["conn"]], ph.num_gp)
+    # planted mistake: drop d(detJ)/dX, i.e. subtract the part of the gradient that scales the whole
+    # element integrand (w.r_e times dN_a/dx_i per Gauss point)
+    dofs = fe.element_dofs(case["conn"], 2)
+    _, r, _ = ph.element(case["coords"][case["conn"]], case["field"][case["conn"]], case["u"][dofs], want_K=False)
+    wr_e = np.einsum("ek,ek->e", r, case["w"][dofs])
+    wrong = gx.copy()
+    for i in range(2):
+        np.add.at(wrong[:, i], case["conn"], -np.einsum("e,eg,ega->ea", wr_e / ph.num_gp ** 2, np.ones_like(detJ), DN[..., i]))
+    v = np.random.default_rng(44).standard_normal(case["coords"].shape)
+    v[:, 2] = 0.0
+    eps = 1e-6
+    fd = (_wR_of_coords(case, case["coords"] + eps * v) - _wR_of_coords(case, case["coords"] - eps * v)) / (2 * eps)
+    assert abs(fd - np.sum(gx * v)) <= 1e-6 * abs(fd)
+    assert abs(fd - np.sum(wrong * v)) > 1e-3 * abs(fd)
