@@ -1,8 +1,10 @@
-"""Mesh container and structured mesh generators (host side, NumPy).
+"""Mesh container, Kratos `.mdpa` reader, legacy VTK writer and structured mesh generators (host
+side, NumPy).
 
-Mirrors the parts of folax's `Mesh` (fol/mesh_input_output/mesh.py:39-181) the hot path uses:
-node coordinates (n_nodes, 3), connectivity per element type (int32), node sets and the orientation
-fix.  File readers (meshio `.med`, Kratos `.mdpa`) are not part of this package.
+Mirrors folax's `Mesh` (fol/mesh_input_output/mesh.py:39-181): node coordinates (n_nodes, 3),
+connectivity per element type (int32), node sets, the orientation fix, `.mdpa` input
+(mesh.py:96-111, 188-242) and `Finalize` export.  The meshio formats (`.med` and the rest) need
+meshio/h5py, which this environment does not have: they raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -11,23 +13,140 @@ import numpy as np
 from . import _lib
 
 
+# Kratos element names -> element type names (the same table as mesh.py:15-37: it is the file
+# format's vocabulary).
+_MDPA_TYPES = {
+    "Line2D2": "line", "Line3D2": "line", "Triangle2D3": "triangle", "Triangle3D3": "triangle",
+    "Quadrilateral2D4": "quad", "Quadrilateral3D4": "quad", "Tetrahedra3D4": "tetra", "Hexahedra3D8": "hexahedron",
+    "Prism3D6": "wedge", "Line2D3": "line3", "Triangle2D6": "triangle6", "Triangle3D6": "triangle6",
+    "Quadrilateral2D9": "quad9", "Quadrilateral3D9": "quad9", "Tetrahedra3D10": "tetra10",
+    "Hexahedra3D27": "hexahedron27", "Point2D": "vertex", "Point3D": "vertex", "Quadrilateral2D8": "quad8",
+    "Quadrilateral3D8": "quad8", "Hexahedra3D20": "hexahedron20",
+}
+
+# legacy VTK cell type ids
+_VTK_CELL = {"vertex": 1, "line": 3, "triangle": 5, "quad": 9, "tetra": 10, "hexahedron": 12, "wedge": 13,
+             "line3": 21, "triangle6": 22, "quad8": 23, "tetra10": 24, "hexahedron20": 25, "quad9": 28,
+             "hexahedron27": 29}
+
+
+def read_mdpa(path, scale_factor=1.0):
+    """Kratos `.mdpa` text: `Begin Nodes` rows `id x y z`, `Begin Elements <Name>` rows
+    `id property n1 .. nk` (1-based), `Begin SubModelPart <name>` with a `SubModelPartNodes` block.
+    Returns (coords (n,3) float64, {type: int32 conn (0-based)}, {set name: int64 node ids}).
+    Node ids are taken in file order (mesh.py:215-217 uses positions, not the ids)."""
+    coords, elems, sets = None, {}, {}
+    with open(path, "r") as f:
+        lines = f.read().splitlines()
+    i, n = 0, len(lines)
+    while i < n:
+        env = lines[i].strip()
+        i += 1
+        if env.startswith("Begin Nodes"):
+            j = i
+            while j < n and "End Nodes" not in lines[j]:
+                j += 1
+            data = np.array(" ".join(lines[i:j]).split(), dtype=np.float64)
+            coords = data.reshape(-1, 4)[:, 1:] * scale_factor
+            i = j + 1
+        elif env.startswith("Begin Elements"):
+            name = env[len("Begin Elements"):].strip()
+            etype = next((v for k, v in _MDPA_TYPES.items() if k in name), None)
+            rows = []
+            while i < n and not lines[i].startswith("End Elements"):
+                vals = [int(v) for v in lines[i].split()]
+                if vals:
+                    rows.append(vals[2:])
+                i += 1
+            i += 1
+            conn = np.asarray(rows, dtype=np.int64).reshape(len(rows), -1) - 1
+            conn = conn.astype(np.int32)
+            elems[etype] = conn if etype not in elems else np.vstack([elems[etype], conn])
+        elif env.startswith("Begin SubModelPart ") and not env.startswith("Begin SubModelPartNodes"):
+            name = env[len("Begin SubModelPart "):].strip()
+            if i < n and lines[i].strip().startswith("Begin SubModelPartNodes"):
+                i += 1
+                ids = []
+                while i < n and not lines[i].strip().startswith("End SubModelPartNodes"):
+                    if lines[i].strip():
+                        ids.append(int(lines[i].strip()) - 1)
+                    i += 1
+                sets[name] = np.asarray(ids, dtype=np.int64)
+    if coords is None:
+        raise ValueError(f"{path}: no `Begin Nodes` block")
+    return coords, elems, sets
+
+
+def write_vtk(path, coords, elements_nodes, point_data=None):
+    """Legacy ASCII VTK unstructured grid with optional scalar / vector point data."""
+    cells = [(t, np.asarray(c)) for t, c in elements_nodes.items() if t in _VTK_CELL and len(c)]
+    ncell = sum(len(c) for _, c in cells)
+    size = sum(c.size + len(c) for _, c in cells)
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 4.2\nfolax_amd mesh\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+        f.write(f"POINTS {coords.shape[0]} double\n")
+        np.savetxt(f, np.asarray(coords, dtype=np.float64).reshape(-1, 3))
+        f.write(f"CELLS {ncell} {size}\n")
+        for _, c in cells:
+            np.savetxt(f, np.hstack([np.full((len(c), 1), c.shape[1]), c]), fmt="%d")
+        f.write(f"CELL_TYPES {ncell}\n")
+        for t, c in cells:
+            np.savetxt(f, np.full(len(c), _VTK_CELL[t]), fmt="%d")
+        if point_data:
+            f.write(f"POINT_DATA {coords.shape[0]}\n")
+            for name, v in point_data.items():
+                v = np.asarray(v, dtype=np.float64).reshape(coords.shape[0], -1)
+                if v.shape[1] == 1:
+                    f.write(f"SCALARS {name} double 1\nLOOKUP_TABLE default\n")
+                    np.savetxt(f, v)
+                else:
+                    f.write(f"VECTORS {name} double\n")
+                    np.savetxt(f, np.hstack([v, np.zeros((v.shape[0], 3 - v.shape[1]))]) if v.shape[1] < 3 else v)
+
+
 class Mesh:
-    """Container with the reference getters (mesh.py:144-172)."""
+    """Container with the reference getters (mesh.py:144-172).  Reads `.mdpa` files on
+    Initialize (mesh.py:59-120), then applies the orientation fix."""
 
     def __init__(self, io_name: str = "mesh", file_name: str = "mesh.none", case_dir: str = ".", scale_factor=1.0):
         self.name = io_name
         self.file_name = file_name
+        self.mesh_format = file_name.split(".")[1] if "." in file_name else ""
+        self.case_dir = case_dir
+        self.scale_factor = scale_factor
         self.node_ids = np.zeros(0, dtype=np.int64)
         self.nodes_coordinates = np.zeros((0, 3))
         self.elements_nodes = {}
         self.node_sets = {}
+        self.point_data = {}
         self.is_initialized = False
+
+    def GetName(self):
+        return self.name
 
     def Initialize(self):
         if self.is_initialized:
             return
+        import os
+        path = os.path.join(self.case_dir, self.file_name)
+        if self.mesh_format == "mdpa":
+            self.nodes_coordinates, self.elements_nodes, self.node_sets = read_mdpa(path, self.scale_factor)
+            self.node_ids = np.arange(self.nodes_coordinates.shape[0])
+        elif os.path.exists(path):
+            raise NotImplementedError(f"reading .{self.mesh_format} needs meshio/h5py, which are not installed")
         self.CheckAndOrientElements()
         self.is_initialized = True
+
+    def __setitem__(self, key, value):
+        self.point_data[key] = np.asarray(value)
+
+    def Finalize(self, export_dir: str = ".", export_format: str = "vtk"):
+        """Writes the mesh and the assigned point data (mesh.py:177-180); legacy VTK only."""
+        import os
+        if export_format != "vtk":
+            raise NotImplementedError("only the legacy VTK format is written (meshio is not installed)")
+        write_vtk(os.path.join(export_dir, self.file_name.split(".")[0] + ".vtk"), self.nodes_coordinates,
+                  self.elements_nodes, self.point_data)
 
     def CheckAndOrientElements(self):
         """Swap local nodes 0 and 1 of elements whose detJ at the one-point rule is negative
