@@ -22,6 +22,7 @@
 
 #include "femhot_chunk.h"
 #include "femhot_chunk_kernels.h"
+#include "femhot_chunk_pipe.h"
 
 namespace femhot {
 
@@ -42,7 +43,7 @@ template <class R> struct ReduceArgs {
   unsigned flags;
 };
 
-template <class R>
+template <class R, int P>
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> A) {
   const bool accumulate = (A.flags & FEMHOT_ACCUMULATE) != 0;
   if ((int)blockIdx.x >= A.nb_nodes) {
@@ -67,26 +68,41 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> A) {
   const long long b = i / A.n_shared;
   const long long k = i - b * A.n_shared;
   const int g = A.sh_nodes[k];
-  const int j0 = A.red_off[k], j1 = A.red_off[k + 1];
+  const int j0 = A.red_off[k], j1 = A.red_off[k + 1];  // a shared node has a slot in >= 2 chunks
   const bool mask = (A.flags & FEMHOT_MASK_DIRICHLET) != 0;
-  const int P = A.P;
+  // Every load a node needs is issued before the first add: its first two slots (consecutive
+  // memory), the Dirichlet mask and, when accumulating, the old outputs.  Further slots (nodes in
+  // 3+ chunks) follow in a loop.  The sum runs over the slots in chunk order.
   if (A.ou) {
-    const R *pu = A.part_u + b * A.n_slots * P;
+    const R *pu = A.part_u + b * A.n_slots * P + (long long)j0 * P;
+    R *o = A.ou + b * A.ous + (long long)g * P;
+    R v0[P], v1[P], old[P];
+    bool masked[P];
+#pragma unroll
     for (int c = 0; c < P; ++c) {
-      R s = R(-0.0);
-      for (int j = j0; j < j1; ++j) s += pu[(long long)j * P + c];
-      const long long dof = (long long)g * P + c;
-      const bool masked = mask && A.dmask[dof];
-      R *o = A.ou + b * A.ous + dof;
-      if (!(masked && accumulate)) *o = masked ? R(0) : (accumulate ? *o + s : s);
+      v0[c] = pu[c];
+      v1[c] = j1 - j0 >= 2 ? pu[P + c] : R(-0.0);
+      masked[c] = mask && A.dmask[(long long)g * P + c];
+      old[c] = accumulate ? o[c] : R(0);
     }
+    R s[P];
+#pragma unroll
+    for (int c = 0; c < P; ++c) s[c] = (R(-0.0) + v0[c]) + v1[c];
+    for (int j = 2; j < j1 - j0; ++j)
+#pragma unroll
+      for (int c = 0; c < P; ++c) s[c] += pu[j * P + c];
+#pragma unroll
+    for (int c = 0; c < P; ++c)
+      if (!(masked[c] && accumulate)) o[c] = masked[c] ? R(0) : (accumulate ? old[c] + s[c] : s[c]);
   }
   if (A.od) {
-    const R *pd = A.part_d + b * A.n_slots;
-    R s = R(-0.0);
-    for (int j = j0; j < j1; ++j) s += pd[j];
+    const R *pd = A.part_d + b * A.n_slots + j0;
     R *o = A.od + b * A.ods + g;
-    *o = accumulate ? *o + s : s;
+    const R v0 = pd[0], v1 = j1 - j0 >= 2 ? pd[1] : R(-0.0);
+    const R old = accumulate ? *o : R(0);
+    R s = (R(-0.0) + v0) + v1;
+    for (int j = 2; j < j1 - j0; ++j) s += pd[j];
+    *o = accumulate ? old + s : s;
   }
 }
 
@@ -103,11 +119,13 @@ __global__ void k_zero_nodes(const int *nodes, long long n, int P, int batch, R 
   if (od) od[b * ods + g] = R(0);
 }
 
-// grid_per_cu 0: one workgroup per work item.  sb: samples per work item (a work item stages its
+// grid_per_cu 0: one workgroup per work item; k > 0: k workgroups per CU; k < 0: -k workgroups in
+// all (persistent).  sb: samples per work item (a work item stages its
 // chunk's plan once and loops over sb samples); 0 picks enough items for about 8 per CU.
 struct ChunkConfig {
   int epc = 1024, block = 256, grid_per_cu = 0, sb = 0;
   bool plain = false;  // the plain path won the autotuner for this call shape
+  bool dma = false;    // FP32 only: the pipelined LDS-DMA kernel (k_chunk_pipe) instead of k_chunk
 };
 
 static std::mutex g_tune_mu;
@@ -149,8 +167,16 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   const int PP = P == 3 ? 4 : P;  // padded per-node record (see k_chunk)
   const bool nodal_gd = od && !elemental && mode != MODE_RES;
   const size_t cap4 = ((size_t)p.max_nodes + 3) & ~(size_t)3;
-  const size_t shm = cap4 * (4 + PP + (mode == MODE_VJP ? PP : 0) + (elemental ? 0 : 1)) * rs +
-                     (size_t)p.max_elems * c->n * (P + (nodal_gd ? 1 : 0)) * rs + (size_t)p.max_elems * c->n * 2 + 16;
+  if (cfg.dma && dtype != FEMHOT_F32) {
+    set_error("run_chunk: the LDS-DMA chunk kernel is FP32 only");
+    return FEMHOT_ERR_UNSUPPORTED;
+  }
+  const size_t shm =
+      cfg.dma ? (size_t)pipe_layout(p.max_nodes, p.max_elems, c->n, P, mode, elemental, ou != nullptr, nodal_gd,
+                                    p.perm_identity)
+                    .total
+              : cap4 * (4 + PP + (mode == MODE_VJP ? PP : 0) + (elemental ? 0 : 1)) * rs +
+                    (size_t)p.max_elems * c->n * (P + (nodal_gd ? 1 : 0)) * rs + (size_t)p.max_elems * c->n * 2 + 16;
   if (shm > lds_limit()) {
     set_error("run_chunk: chunk too large for LDS; use fewer elements per chunk");
     return FEMHOT_ERR_UNSUPPORTED;
@@ -235,12 +261,22 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   const long long items = (long long)A.n_chunks * A.n_sblocks;
   long long grid = items;
   if (cfg.grid_per_cu > 0) grid = std::min<long long>(items, (long long)cfg.grid_per_cu * cu_count());
+  if (cfg.grid_per_cu < 0) grid = std::min<long long>(items, -(long long)cfg.grid_per_cu);
   if (grid < 1) return FEMHOT_OK;
   if ((rc = timing_mark(c, s))) return rc;
-  switch (c->et) {
-    case QUAD4: rc = launch_chunk_et<QUAD4>(dtype, c->ph, c->ngp, mode, cfg.block, dim3((unsigned)grid), shm, s, A); break;
-    case HEX8: rc = launch_chunk_et<HEX8>(dtype, c->ph, c->ngp, mode, cfg.block, dim3((unsigned)grid), shm, s, A); break;
-    default: rc = launch_chunk_et<TET4>(dtype, c->ph, c->ngp, mode, cfg.block, dim3((unsigned)grid), shm, s, A); break;
+  const dim3 gd((unsigned)grid);
+  if (cfg.dma) {
+    switch (c->et) {
+      case QUAD4: rc = launch_chunk_pipe_et<QUAD4>(c->ph, c->ngp, mode, cfg.block, gd, shm, s, A); break;
+      case HEX8: rc = launch_chunk_pipe_et<HEX8>(c->ph, c->ngp, mode, cfg.block, gd, shm, s, A); break;
+      default: rc = launch_chunk_pipe_et<TET4>(c->ph, c->ngp, mode, cfg.block, gd, shm, s, A); break;
+    }
+  } else {
+    switch (c->et) {
+      case QUAD4: rc = launch_chunk_et<QUAD4>(dtype, c->ph, c->ngp, mode, cfg.block, gd, shm, s, A); break;
+      case HEX8: rc = launch_chunk_et<HEX8>(dtype, c->ph, c->ngp, mode, cfg.block, gd, shm, s, A); break;
+      default: rc = launch_chunk_et<TET4>(dtype, c->ph, c->ngp, mode, cfg.block, gd, shm, s, A); break;
+    }
   }
   if (rc) return rc;
   // deterministic reductions of shared nodes and per-chunk energies (one launch)
@@ -274,7 +310,13 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
         ra.chunk_energy = chunk_energy;
         ra.energy = energy;
         ra.flags = flags;
-        hipLaunchKernelGGL(k_reduce<R>, dim3((unsigned)(nb_nodes + nb_energy)), dim3(256), 0, s, ra);
+        const dim3 rg((unsigned)(nb_nodes + nb_energy));
+        if (P == 1)
+          hipLaunchKernelGGL((k_reduce<R, 1>), rg, dim3(256), 0, s, ra);
+        else if (P == 2)
+          hipLaunchKernelGGL((k_reduce<R, 2>), rg, dim3(256), 0, s, ra);
+        else
+          hipLaunchKernelGGL((k_reduce<R, 3>), rg, dim3(256), 0, s, ra);
         FH_HIP(hipGetLastError());
         return FEMHOT_OK;
       };
@@ -328,8 +370,16 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
                    double *energy, unsigned flags, hipStream_t s, int *best_epc, int *best_block, int *best_grid,
                    float *best_ms) {
   static const int epcs[] = {128, 256, 512, 1024, 2048};
-  static const int blocks[] = {128, 256, 512};
-  static const int grids[] = {0, 4, 8};
+  // staged kernel: 3 workgroup sizes x {one workgroup per item, 4 or 8 per CU}; pipelined kernel
+  // (FP32): 2 workgroup sizes x persistent grids only (its prefetch needs a next step)
+  struct Shape {
+    bool dma;
+    int block, grid;
+  };
+  static const Shape shapes[] = {{false, 128, 0}, {false, 128, 4}, {false, 128, 8}, {false, 256, 0},
+                                 {false, 256, 4}, {false, 256, 8}, {false, 512, 0}, {false, 512, 4},
+                                 {false, 512, 8}, {true, 256, 1},  {true, 256, 2},  {true, 256, 4},
+                                 {true, 512, 1},  {true, 512, 2},  {true, 512, 4}};
   const int sbs[] = {0, 1};  // automatic, and one sample per work item (only differs when batch > 1)
   hipEvent_t e0, e1;
   FH_HIP(hipEventCreate(&e0));
@@ -338,34 +388,35 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
   float best_t = 1e30f;
   for (int epc : epcs) {
     if ((long long)epc > c->Ne * 2 && epc != 128) continue;
-    for (int blk : blocks)
-      for (int gp : grids)
-        for (int sb : sbs) {
-          if (sb == 1 && batch == 1) continue;
-          ChunkConfig cfg;
-          cfg.epc = epc;
-          cfg.block = blk;
-          cfg.grid_per_cu = gp;
-          cfg.sb = sb;
-          int rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, s, cfg);
-          if (rc == FEMHOT_ERR_UNSUPPORTED) continue;  // chunk does not fit LDS
+    for (const Shape &sh : shapes)
+      for (int sb : sbs) {
+        if (sb == 1 && batch == 1) continue;
+        if (sh.dma && dtype != FEMHOT_F32) continue;
+        ChunkConfig cfg;
+        cfg.epc = epc;
+        cfg.block = sh.block;
+        cfg.grid_per_cu = sh.grid;
+        cfg.dma = sh.dma;
+        cfg.sb = sb;
+        int rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, s, cfg);
+        if (rc == FEMHOT_ERR_UNSUPPORTED) continue;  // chunk does not fit LDS
+        if (rc) return rc;
+        FH_HIP(hipEventRecord(e0, s));
+        constexpr int reps = 3;
+        for (int r = 0; r < reps; ++r) {
+          rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, s, cfg);
           if (rc) return rc;
-          FH_HIP(hipEventRecord(e0, s));
-          constexpr int reps = 3;
-          for (int r = 0; r < reps; ++r) {
-            rc = run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, w, ws, ou, ous, od, ods, energy, flags, s, cfg);
-            if (rc) return rc;
-          }
-          FH_HIP(hipEventRecord(e1, s));
-          FH_HIP(hipEventSynchronize(e1));
-          float ms = 0.f;
-          FH_HIP(hipEventElapsedTime(&ms, e0, e1));
-          ms /= reps;
-          if (ms < best_t) {
-            best_t = ms;
-            best = cfg;
-          }
         }
+        FH_HIP(hipEventRecord(e1, s));
+        FH_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= reps;
+        if (ms < best_t) {
+          best_t = ms;
+          best = cfg;
+        }
+      }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -380,21 +431,26 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
   int rc = ensure_chunks(c, best.epc);  // leave the winning plan resident
   if (rc) return rc;
   if (best_epc) *best_epc = best.epc;
-  if (best_block) *best_block = best.block;
+  if (best_block) *best_block = best.block | (best.dma ? (int)FEMHOT_CHUNK_DMA : 0);
   if (best_grid) *best_grid = best.grid_per_cu;
   if (best_ms) *best_ms = best_t;
   return FEMHOT_OK;
 }
 
 int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int epc, int block, int grid_per_cu) {
-  if (epc < 1 || epc > 8192 || (block != 128 && block != 256 && block != 512) || grid_per_cu < 0) {
-    set_error("femhot_set_chunk_config: epc in [1,8192], block in {128,256,512}, grid_per_cu >= 0");
+  const bool dma = (block & (int)FEMHOT_CHUNK_DMA) != 0;
+  block &= ~(int)FEMHOT_CHUNK_DMA;
+  if (epc < 1 || epc > 8192 || (block != 128 && block != 256 && block != 512) ||
+      (dma && (dtype != FEMHOT_F32 || block == 128))) {
+    set_error("femhot_set_chunk_config: epc in [1,8192], block in {128,256,512} (FEMHOT_CHUNK_DMA: FP32, "
+              "{256,512})");
     return FEMHOT_ERR_ARG;
   }
   ChunkConfig cfg;
   cfg.epc = epc;
   cfg.block = block;
   cfg.grid_per_cu = grid_per_cu;
+  cfg.dma = dma;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[{c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)}] = cfg;

@@ -54,14 +54,30 @@ template <int N> __device__ __forceinline__ void load_lconn(const uint16_t *p, i
 // 16-byte ds access; PP is the padded stride.
 template <int P> struct Pad { static constexpr int PP = (P == 3) ? 4 : P; };
 
+// An element's K per-corner results (K a multiple of 16 bytes) written with 16-byte LDS stores.
+// Consecutive lanes hold consecutive elements, so with K * sizeof(R) = 48 / 96 bytes each group
+// of 8 lanes covers 8 distinct 4-bank groups.
+template <class R, int K> __device__ __forceinline__ void store_corner_results(R *dst, const R (&v)[K]) {
+  static_assert((K * sizeof(R)) % 16 == 0, "per-element record must be whole 16-byte units");
+  if constexpr (sizeof(R) == 4) {
+#pragma unroll
+    for (int i = 0; i < K; i += 4)
+      reinterpret_cast<float4 *>(dst)[i / 4] = make_float4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < K; i += 2) reinterpret_cast<double2 *>(dst)[i / 2] = make_double2(v[i], v[i + 1]);
+  }
+}
+
 // LDS layout of k_chunk (in units of R unless noted; cap4 = max chunk nodes rounded up to 4,
 // emax = max chunk elements):
 //   X   [cap4 * 4]      chunk-relative coordinates (x, y, z, 0)
 //   U   [cap4 * PP]     nodal solution
 //   W   [cap4 * PP]     cotangent (VJP only)
 //   FD  [cap4]          nodal field (nodal fields only)
-//   ER  [P * N * emax]  per element corner, component- and corner-major ([c][a][position]): residual / g_u
-//   ERD [N * emax]      per element corner, corner-major: nodal field-gradient contribution
+//   ER  [emax * N * P]  per element corner ([position][corner][component]): residual / g_u.  An
+//                       incidence entry q = position * N + corner addresses its P values at q * P.
+//   ERD [emax * N]      per element corner ([position][corner]): nodal field-gradient contribution
 //   INC [emax * N] (uint16) the chunk's incidence list
 template <class R, int ET, int PH, int NGP, int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
@@ -69,7 +85,6 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL, PP = Pad<P>::PP;
   constexpr int KN = 4;                                  // nodes per lane per round
   constexpr int KE = (ET != HEX8 && NGP <= 2) ? 4 : 1;  // elements per lane per round
-  constexpr int LOGN = N == 4 ? 2 : 3;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[BLOCK / 64];
   const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
@@ -226,12 +241,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             elem_egrad<R, ET, PH, NGP>(el, A.pp, ro, gd, pi);
             en += (double)pi;
           }
-          if (want_ou) {  // component- and corner-major: consecutive lanes write consecutive words
-#pragma unroll
-            for (int a = 0; a < N; ++a)
-#pragma unroll
-              for (int c = 0; c < P; ++c) ER[(c * N + a) * emax + pos] = ro[a * P + c];
-          }
+          if (want_ou) store_corner_results<R, NL>(ER + (long)pos * NL, ro);
           if constexpr (MODE != MODE_RES) {
             if (want_gd) {
               if (elemental) {
@@ -240,8 +250,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
                 else
                   ODb[ee[k]] = gd[0];
               } else {
-#pragma unroll
-                for (int a = 0; a < N; ++a) ERD[a * emax + pos] = gd[a];
+                store_corner_results<R, N>(ERD + (long)pos * N, gd);
               }
             }
           }
@@ -282,10 +291,9 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
 #pragma unroll
               for (int c = 0; c < P; ++c) s[c] = R(-0.0);
               for (int j = js[k]; j < je[k]; ++j) {
-                const int q = INC[j];
-                const int r = (q & (N - 1)) * emax + (q >> LOGN);
+                const R *e = ER + INC[j] * P;
 #pragma unroll
-                for (int c = 0; c < P; ++c) s[c] += ER[c * N * emax + r];
+                for (int c = 0; c < P; ++c) s[c] += e[c];
               }
 #pragma unroll
               for (int c = 0; c < P; ++c) {
@@ -305,10 +313,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             }
             if (nodal_gd) {
               R s = R(0);
-              for (int j = js[k]; j < je[k]; ++j) {
-                const int q = INC[j];
-                s += ERD[(q & (N - 1)) * emax + (q >> LOGN)];
-              }
+              for (int j = js[k]; j < je[k]; ++j) s += ERD[INC[j]];
               if (sl[k] >= 0)
                 PD[sl[k]] = s;
               else if (accumulate)

@@ -372,6 +372,9 @@ int ensure_chunks(Ctx *c, int epc) {
   p.n_shared = (long long)sh_nodes.size();
   p.n_zero = (long long)zero.size();
   p.n_slots = (long long)red_off.back();
+  // the pipelined kernel reads inc_start as whole 32-bit words: one entry of padding past the end
+  inc_start.push_back(0);
+  inc_start.push_back(0);
   int rc;
   if ((rc = upload(&p.chunk_elem_off, elem_off)) || (rc = upload(&p.chunk_node_off, node_off)) ||
       (rc = upload(&p.chunk_seg, seg)) || (rc = upload(&p.perm, perm)) || (rc = upload(&p.lconn, lconn)) ||

@@ -159,7 +159,10 @@ int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *c
  * chunk path except in FP64 Neo-Hookean, which takes the plain path.  Finally runs the operation
  * once more so the outputs are valid.  Synchronises the stream (not for capture).
  * op: FEMHOT_OP_RESIDUAL / FEMHOT_OP_VJP / FEMHOT_OP_ENERGY_GRAD.  No counterpart in the
- * reference (XLA fuses the element vmap itself); it replaces fe_loss.py:105-114's fixed batch size. */
+ * reference (XLA fuses the element vmap itself); it replaces fe_loss.py:105-114's fixed batch size.
+ * In FP32 the candidates include the software-pipelined chunk kernel, whose staging is LDS DMA
+ * issued one step ahead of the arithmetic; *block then carries FEMHOT_CHUNK_DMA. */
+#define FEMHOT_CHUNK_DMA 0x10000
 #define FEMHOT_OP_RESIDUAL 0
 #define FEMHOT_OP_VJP 1
 #define FEMHOT_OP_ENERGY_GRAD 2
@@ -170,7 +173,10 @@ int femhot_autotune(femhot_ctx *ctx, int op, int dtype, int batch, const void *f
 
 /* Sets the chunk configuration femhot_autotune would record (e.g. one saved from an earlier
  * process) for (ctx, op, dtype, batch, field location) and builds its chunk plan now, so later
- * calls (and graph captures) allocate nothing.  No counterpart in the reference. */
+ * calls (and graph captures) allocate nothing.  block: 128 / 256 / 512, or 256 / 512 |
+ * FEMHOT_CHUNK_DMA for the pipelined LDS-DMA kernel (FP32 only).  grid_per_cu: 0 = one workgroup
+ * per work item, k > 0 = k persistent workgroups per CU, k < 0 = -k persistent workgroups in all.
+ * No counterpart in the reference. */
 int femhot_set_chunk_config(femhot_ctx *ctx, int op, int dtype, int batch, unsigned flags, int epc, int block,
                             int grid_per_cu);
 

@@ -99,7 +99,7 @@ def test_chunk_batched_accumulate_and_epc(dt):
         assert rel(acc_d[b].cpu().numpy() - 2.0, g2) <= 10 * TOL[dt]
     # autotuner keeps results valid and the tuned plan gives the same answer
     epc, blk, grid, ms = ctx.autotune(L.OP_VJP, dev(F, dt), dev(U, dt), dev(W, dt))
-    assert epc >= 256 and blk in (128, 256, 512) and ms > 0
+    assert epc >= 256 and (blk & ~L.CHUNK_DMA) in (128, 256, 512) and ms > 0
     gu2, gd2 = ctx.residual_vjp(dev(F, dt), dev(U, dt), dev(W, dt), flags=CHUNK)
     torch.cuda.synchronize()
     assert rel(gu2.cpu().numpy(), gu1.cpu().numpy()) <= TOL[dt]
