@@ -30,6 +30,24 @@ struct ChunkArgs {
   PhysParams pp;
 };
 
+template <int N, class UN> __device__ __forceinline__ void unpack_lconn(const UN &v, int (&l)[N]) {
+  if constexpr (N == 4) {
+    l[0] = v.x & 0xffff;
+    l[1] = v.x >> 16;
+    l[2] = v.y & 0xffff;
+    l[3] = v.y >> 16;
+  } else {
+    l[0] = v.x & 0xffff;
+    l[1] = v.x >> 16;
+    l[2] = v.y & 0xffff;
+    l[3] = v.y >> 16;
+    l[4] = v.z & 0xffff;
+    l[5] = v.z >> 16;
+    l[6] = v.w & 0xffff;
+    l[7] = v.w >> 16;
+  }
+}
+
 template <int N> __device__ __forceinline__ void load_lconn(const uint16_t *p, int (&l)[N]) {
   if constexpr (N == 4) {
     const uint2 v = *reinterpret_cast<const uint2 *>(p);
@@ -83,8 +101,10 @@ template <class R, int ET, int PH, int NGP, int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL, PP = Pad<P>::PP;
-  constexpr int KN = 4;                                  // nodes per lane per round
-  constexpr int KE = (ET != HEX8 && NGP <= 2) ? 4 : 1;  // elements per lane per round
+  // nodes / elements per lane per round of loads: enough that one round covers 512 nodes and
+  // 1024 elements (a 1024-element chunk's nodes and elements) without idle registers at 512 threads
+  constexpr int KN = BLOCK >= 512 ? 1 : (BLOCK == 256 ? 2 : 4);
+  constexpr int KE = (ET != HEX8 && NGP <= 2) ? (BLOCK >= 512 ? 2 : 4) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[BLOCK / 64];
   const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
@@ -104,12 +124,25 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   uint16_t *INC = reinterpret_cast<uint16_t *>(ERD + (nodal_gd ? (long)emax * N : 0));
   const int tid = threadIdx.x;
   const int n_items = A.n_chunks * A.n_sblocks;
+  using UN = typename std::conditional<N == 4, uint2, uint4>::type;  // one element's N uint16
   for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
     const int chunk = item / A.n_sblocks;
     const int sbi = item - chunk * A.n_sblocks;
     const int e0 = A.elem_off[chunk], e1 = A.elem_off[chunk + 1];
     const int ne = e1 - e0;
     const int n0 = A.node_off[chunk], nl = A.node_off[chunk + 1] - n0;
+    // The first round of this lane's nodes: id, partial slot and incidence range, loaded with the
+    // plan and kept in registers for the solution gather and the write-out (indices clamped, so
+    // the loads of a round issue back to back).
+    int g0[KN], sl0[KN], js0[KN], je0[KN];
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+      const int i = min(k * BLOCK + tid, nl - 1);
+      g0[k] = A.nodes[n0 + i] & 0x7fffffff;
+      sl0[k] = A.node_slot[n0 + i];
+      js0[k] = A.inc_start[n0 + i];
+      je0[k] = (i + 1 < nl) ? (int)A.inc_start[n0 + i + 1] : N * ne;
+    }
     {  // coordinates (one 16-/32-byte record per node) and the incidence list
       const R *xyz = reinterpret_cast<const R *>(A.xyz) + 4LL * n0;
       for (int i = tid; i < nl; i += BLOCK) {
@@ -120,9 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
           reinterpret_cast<double2 *>(X)[2 * i + 1] = reinterpret_cast<const double2 *>(xyz)[2 * i + 1];
         }
       }
-      if (want_ou || nodal_gd) {
-        // one element's worth (N entries, 8 or 16 bytes) per access
-        using UN = typename std::conditional<N == 4, uint2, uint4>::type;
+      if (want_ou || nodal_gd) {  // one element's worth (N entries, 8 or 16 bytes) per access
         const UN *inc = reinterpret_cast<const UN *>(A.inc + (long long)N * e0);
         for (int j = tid; j < ne; j += BLOCK) reinterpret_cast<UN *>(INC)[j] = inc[j];
       }
@@ -132,22 +163,9 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       const R *Ub = reinterpret_cast<const R *>(A.u) + (long long)b * A.us;
       const R *Wb = reinterpret_cast<const R *>(A.w) + (long long)b * A.ws;
       const R *Fb = reinterpret_cast<const R *>(A.field) + (long long)b * A.fs;
-      // stage nodal solution / cotangent / field; indices are clamped (not branched on) so every
-      // load of a round is issued back to back
-      for (int base = 0; base < nl; base += KN * BLOCK) {
-        int g[KN];
-#pragma unroll
-        for (int k = 0; k < KN; ++k) g[k] = A.nodes[n0 + min(base + k * BLOCK + tid, nl - 1)] & 0x7fffffff;
-        R uv[KN][P], wv[KN][P], fv[KN];
-#pragma unroll
-        for (int k = 0; k < KN; ++k) {
-#pragma unroll
-          for (int c = 0; c < P; ++c) {
-            uv[k][c] = Ub[(long long)g[k] * P + c];
-            if constexpr (MODE == MODE_VJP) wv[k][c] = Wb[(long long)g[k] * P + c];
-          }
-          fv[k] = elemental ? R(0) : Fb[g[k]];
-        }
+      R uv[KN][P], wv[KN][P], fv[KN];
+      // stage nodal solution / cotangent / field of one round of nodes into LDS
+      auto store_round = [&](int base) {
 #pragma unroll
         for (int k = 0; k < KN; ++k) {
           const int i = base + k * BLOCK + tid;
@@ -160,6 +178,43 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             if (!elemental) FD[i] = fv[k];
           }
         }
+      };
+      auto load_round = [&](const int (&g)[KN]) {
+#pragma unroll
+        for (int k = 0; k < KN; ++k) {
+#pragma unroll
+          for (int c = 0; c < P; ++c) {
+            uv[k][c] = Ub[(long long)g[k] * P + c];
+            if constexpr (MODE == MODE_VJP) wv[k][c] = Wb[(long long)g[k] * P + c];
+          }
+          fv[k] = elemental ? R(0) : Fb[g[k]];
+        }
+      };
+      load_round(g0);
+      // the first round of elements' local connectivity, original ids and per-element field, in
+      // flight together with the solution gather above
+      UN lw0[KE];
+      int ee0[KE];
+      R dev0[KE];
+#pragma unroll
+      for (int k = 0; k < KE; ++k) {
+        const int p = min(e0 + k * BLOCK + tid, e1 - 1);
+        lw0[k] = *reinterpret_cast<const UN *>(A.lconn + (long long)p * N);
+        ee0[k] = A.perm_identity ? p : A.perm[p];
+      }
+#pragma unroll
+      for (int k = 0; k < KE; ++k) dev0[k] = R(0);
+      if (elemental) {
+#pragma unroll
+        for (int k = 0; k < KE; ++k) dev0[k] = Fb[ee0[k]];
+      }
+      store_round(0);
+      for (int base = KN * BLOCK; base < nl; base += KN * BLOCK) {  // chunks of more nodes
+        int g[KN];
+#pragma unroll
+        for (int k = 0; k < KN; ++k) g[k] = A.nodes[n0 + min(base + k * BLOCK + tid, nl - 1)] & 0x7fffffff;
+        load_round(g);
+        store_round(base);
       }
       __syncthreads();
       double en = 0.0;
@@ -167,15 +222,24 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       for (int base = e0; base < e1; base += KE * BLOCK) {
         int l[KE][N], ee[KE];
         R dev[KE];
+        if (base == e0) {
 #pragma unroll
-        for (int k = 0; k < KE; ++k) {
-          const int p = min(base + k * BLOCK + tid, e1 - 1);
-          load_lconn<N>(A.lconn + (long long)p * N, l[k]);
-          ee[k] = A.perm_identity ? p : A.perm[p];
-        }
-        if (elemental) {
+          for (int k = 0; k < KE; ++k) {
+            unpack_lconn<N>(lw0[k], l[k]);
+            ee[k] = ee0[k];
+            dev[k] = dev0[k];
+          }
+        } else {
 #pragma unroll
-          for (int k = 0; k < KE; ++k) dev[k] = Fb[ee[k]];
+          for (int k = 0; k < KE; ++k) {
+            const int p = min(base + k * BLOCK + tid, e1 - 1);
+            load_lconn<N>(A.lconn + (long long)p * N, l[k]);
+            ee[k] = A.perm_identity ? p : A.perm[p];
+          }
+          if (elemental) {
+#pragma unroll
+            for (int k = 0; k < KE; ++k) dev[k] = Fb[ee[k]];
+          }
         }
 #pragma unroll
         for (int k = 0; k < KE; ++k) {
@@ -274,13 +338,18 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       if (want_ou || nodal_gd) {
         for (int base = 0; base < nl; base += KN * BLOCK) {
           int g[KN], sl[KN], js[KN], je[KN];
+          if (base == 0) {
 #pragma unroll
-          for (int k = 0; k < KN; ++k) {
-            const int i = min(base + k * BLOCK + tid, nl - 1);
-            g[k] = A.nodes[n0 + i] & 0x7fffffff;
-            sl[k] = A.node_slot[n0 + i];
-            js[k] = A.inc_start[n0 + i];
-            je[k] = (i + 1 < nl) ? (int)A.inc_start[n0 + i + 1] : N * ne;
+            for (int k = 0; k < KN; ++k) g[k] = g0[k], sl[k] = sl0[k], js[k] = js0[k], je[k] = je0[k];
+          } else {
+#pragma unroll
+            for (int k = 0; k < KN; ++k) {
+              const int i = min(base + k * BLOCK + tid, nl - 1);
+              g[k] = A.nodes[n0 + i] & 0x7fffffff;
+              sl[k] = A.node_slot[n0 + i];
+              js[k] = A.inc_start[n0 + i];
+              je[k] = (i + 1 < nl) ? (int)A.inc_start[n0 + i + 1] : N * ne;
+            }
           }
 #pragma unroll
           for (int k = 0; k < KN; ++k) {
