@@ -24,8 +24,8 @@ void forget_tuning(const void *ctx);
 
 // True when a call of this shape should take the plain path even though FEMHOT_SCATTER_CHUNK is set:
 // the autotuner found the plain path faster, or, untuned, the call is FP64 Neo-Hookean (its element
-// routine needs the whole register file, which leaves the chunk kernel at one wave per SIMD; on
-// HEX8 48^3 x 16 the plain path takes 1.04 ms against 1.4-2.1 ms).
+// routine needs most of the register file, so only small chunks with small workgroups keep the chunk
+// kernel ahead; femhot_autotune times both and records the winner).
 bool chunk_prefers_plain(const Ctx *c, int dtype, int mode, int batch, unsigned flags);
 
 // Records that the plain path won the autotuner for this call shape.
