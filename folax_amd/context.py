@@ -250,6 +250,26 @@ class FemhotContext:
         L.check(self._lib.femhot_spmv(self._h, _ptr(row_ptr), _ptr(col_idx), _ptr(vals), _ptr(x), _ptr(y), _stream()))
         return y
 
+    def newton(self, field, u, load_steps=1, max_iter=20, abs_tol=1e-8, rel_tol=1e-8, lin_tol=1e-10,
+               lin_maxiter=10000, mode="f64", flags=0):
+        """Load-stepped Newton solve on the device (femhot_newton).  field, u: FP64 device vectors; u is
+        the initial guess.  Returns (u, history (load_steps, max_iter, 2) of (||R||, ||du||) with NaN
+        after each step's last iteration, iterations per load step).  Raises FemhotError with
+        ERR_NONFINITE when the residual norm becomes NaN."""
+        if u.dtype != torch.float64 or field.dtype != torch.float64:
+            raise TypeError("newton expects FP64 field and dofs")
+        field = field.reshape(-1).contiguous()
+        x = u.reshape(-1).clone()
+        self._check_in(x.view(1, -1), "u", self.n_dofs, 1, x.dtype, x.device)
+        self._check_in(field.view(1, -1), "field", self._field_size(flags), 1, x.dtype, x.device)
+        hist = (C.c_double * (2 * load_steps * max_iter))()
+        steps = (C.c_int * load_steps)()
+        m = L.F64 if mode == "f64" else L.F32
+        L.check(self._lib.femhot_newton(self._h, m, _ptr(field), _ptr(x), int(flags), int(load_steps), int(max_iter),
+                                        float(abs_tol), float(rel_tol), float(lin_tol), int(lin_maxiter), hist, steps,
+                                        _stream()))
+        return x, np.array(hist[:], dtype=np.float64).reshape(load_steps, max_iter, 2), list(steps)
+
     def pcg(self, row_ptr, col_idx, vals, b, x0=None, tol=1e-10, maxiter=10000, mode="f64", raise_on_fail=True):
         """Jacobi-PCG (FP64), or FP32 inner iterations + FP64 refinement (mode='mixed')."""
         if vals.dtype != torch.float64 or b.dtype != torch.float64:

@@ -582,6 +582,7 @@ void femhot_ctx_destroy(femhot_ctx *h) {
   (void)hipFree(c.d_colour_elems);
   free_chunks(c.chunk);
   (void)hipFree(c.chunk_ws.ptr);
+  (void)hipFree(c.newton_ws.ptr);
   (void)hipFree(c.d_ws);
   for (hipEvent_t e : c.tev_pool) (void)hipEventDestroy(e);
   if (c.h_pinned) (void)hipHostFree(c.h_pinned);

@@ -103,6 +103,8 @@ struct Ctx {
   int *d_colour_elems = nullptr;  // elements sorted by colour (stable)
   ChunkPlan chunk;                // built on demand
   ChunkWork chunk_ws;
+  ChunkWork newton_ws;            // femhot_newton: tangent values, residual, rhs, step, Dirichlet values
+  bool newton_dval_ready = false;
   // kernel timing hook: events recorded around the dominant kernel of each call while enabled
   bool timing = false;
   std::vector<hipEvent_t> tev_pool;

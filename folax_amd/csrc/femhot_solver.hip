@@ -395,3 +395,111 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
   }
   return FEMHOT_OK;
 }
+
+// ------------------------------------------------------------------------------------------
+// Load-stepped Newton-Raphson driver (fol/solvers/fe_nonlinear_residual_based_solver.py:76-170)
+// ------------------------------------------------------------------------------------------
+
+// Dirichlet dofs of u set to scale * value (ApplyDirichletBCOnDofVector, fe_loss.py:91-92 with the
+// load increment of the nonlinear solver).
+__global__ void k_apply_dirichlet(long long n, double *u, const uint8_t *dmask, const double *dval, double scale) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && dmask[i]) u[i] = scale * dval[i];
+}
+__global__ void k_newton_rhs(long long n, const double *R, double *b, double *du) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    b[i] = -R[i];
+    du[i] = 0.0;
+  }
+}
+__global__ void k_add64(long long n, double *u, const double *du) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] += du[i];
+}
+
+extern "C" int femhot_newton(femhot_ctx *h, int mode, const double *field, double *u, unsigned flags,
+                             int load_steps, int max_iter, double abs_tol, double rel_tol, double lin_tol,
+                             int lin_maxiter, double *history, int *steps, void *stream) {
+  FH_CHECK_ARG(h && field && u, FEMHOT_ERR_ARG, "femhot_newton: null argument");
+  FH_CHECK_ARG(mode == FEMHOT_F64 || mode == FEMHOT_F32, FEMHOT_ERR_ARG,
+               "femhot_newton: mode must be FEMHOT_F64 or FEMHOT_F32");
+  FH_CHECK_ARG(load_steps > 0 && max_iter > 0 && lin_tol > 0 && lin_maxiter > 0, FEMHOT_ERR_ARG,
+               "femhot_newton: load_steps, max_iter, lin_tol and lin_maxiter must be positive");
+  FH_CHECK_ARG((flags & ~(FEMHOT_ELEMENT_FIELD | FEMHOT_SCATTER_COLOUR)) == 0, FEMHOT_ERR_ARG,
+               "femhot_newton: flags may hold FEMHOT_ELEMENT_FIELD and FEMHOT_SCATTER_COLOUR only");
+  Ctx *c = &h->c;
+  int rc = ensure_pattern(c);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const long long n = c->Ndof, nnz = c->nnz;
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t need = up(sizeof(double) * std::max<long long>(nnz, 1)) + 4 * up(sizeof(double) * n) +
+                      up(sizeof(double) * NBR);
+  if (c->newton_ws.bytes < need) {
+    FH_HIP(hipStreamSynchronize(s));
+    if (c->newton_ws.ptr) FH_HIP(hipFree(c->newton_ws.ptr));
+    c->newton_ws.ptr = nullptr;
+    c->newton_ws.bytes = 0;
+    c->newton_dval_ready = false;
+    FH_HIP(hipMalloc(&c->newton_ws.ptr, need));
+    c->newton_ws.bytes = need;
+  }
+  char *ws = (char *)c->newton_ws.ptr;
+  auto take = [&](size_t bytes) {
+    char *p = ws;
+    ws += up(bytes);
+    return (void *)p;
+  };
+  double *vals = (double *)take(sizeof(double) * std::max<long long>(nnz, 1));
+  double *R = (double *)take(sizeof(double) * n), *b = (double *)take(sizeof(double) * n);
+  double *du = (double *)take(sizeof(double) * n), *dval = (double *)take(sizeof(double) * n);
+  double *part = (double *)take(sizeof(double) * NBR);
+  if (!c->newton_dval_ready) {  // Dirichlet values as a dense dof vector (the last entry of a repeated dof wins)
+    std::vector<double> hv((size_t)n, 0.0);
+    for (size_t k = 0; k < c->h_dir_idx.size(); ++k) hv[(size_t)c->h_dir_idx[k]] = c->h_dir_val[k];
+    FH_HIP(hipStreamSynchronize(s));
+    FH_HIP(hipMemcpy(dval, hv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    c->newton_dval_ready = true;
+  }
+  if (history)
+    for (long long k = 0; k < 2LL * load_steps * max_iter; ++k) history[k] = std::nan("");
+  const dim3 grid((unsigned)((n + 255) / 256));
+  for (int step = 1; step <= load_steps; ++step) {
+    hipLaunchKernelGGL(k_apply_dirichlet, grid, dim3(256), 0, s, n, u, c->d_dmask, dval,
+                       (double)step / load_steps);
+    int it = 1;
+    for (;; ++it) {
+      rc = femhot_tangent_csr(h, FEMHOT_F64, field, u, vals, R, flags, stream);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_dot64, dim3(NBR), dim3(256), 0, s, n, R, R, part);
+      double rr = 0.0;
+      if ((rc = host_sum_partials(c, part, s, &rr))) return rc;
+      const double res_norm = std::sqrt(rr);
+      if (std::isnan(res_norm)) {
+        if (steps) steps[step - 1] = it;
+        set_error("femhot_newton: residual norm contains NaN values");
+        return FEMHOT_ERR_NONFINITE;
+      }
+      hipLaunchKernelGGL(k_newton_rhs, grid, dim3(256), 0, s, n, R, b, du);
+      int lit = 0;
+      double lrel = 0.0;
+      rc = femhot_pcg(h, mode, c->d_row_ptr, c->d_col_idx, vals, b, du, lin_tol, lin_maxiter, &lit, &lrel, stream);
+      if (rc && rc != FEMHOT_ERR_NOT_CONVERGED) return rc;  // an unconverged solve continues with its iterate
+      hipLaunchKernelGGL(k_dot64, dim3(NBR), dim3(256), 0, s, n, du, du, part);
+      double dd = 0.0;
+      if ((rc = host_sum_partials(c, part, s, &dd))) return rc;
+      const double du_norm = std::sqrt(dd);
+      if (history) {
+        history[2LL * ((long long)(step - 1) * max_iter + (it - 1))] = res_norm;
+        history[2LL * ((long long)(step - 1) * max_iter + (it - 1)) + 1] = du_norm;
+      }
+      if (res_norm < abs_tol || du_norm < rel_tol || it == max_iter) break;
+      hipLaunchKernelGGL(k_add64, grid, dim3(256), 0, s, n, u, du);
+    }
+    if (steps) steps[step - 1] = it;
+  }
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipStreamSynchronize(s));
+  return FEMHOT_OK;
+}

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import torch
 
+from . import _lib as L
 from .loss_functions import FiniteElementLoss, _as_device_tensor
 
 
@@ -76,23 +77,27 @@ class FiniteElementNonLinearResidualBasedSolver(FiniteElementLinearResidualBased
                                                      self.fe_solver_settings["nonlinear_solver_settings"])
 
     def Solve(self, current_control_vars, current_dofs_np):
-        u = _as_device_tensor(current_dofs_np)
+        """The load-stepped Newton loop runs inside the library (femhot_newton): assembly, norms, the
+        NaN guard, the linear solve and the update stay on the device."""
+        loss = self.fe_loss_function
+        u = _as_device_tensor(current_dofs_np).to(torch.float64)
+        d = _as_device_tensor(current_control_vars, dtype=torch.float64)
         s = self.nonlinear_solver_settings
-        self.convergence_history = {}
-        for step in range(1, s["load_incr"] + 1):
-            u = self.fe_loss_function.ApplyDirichletBCOnDofVector(u, step / s["load_incr"])
-            hist = {"res_norm": [], "delta_dofs_norm": []}
-            self.convergence_history[step] = hist
-            for i in range(1, s["maxiter"] + 1):
-                K, R = self.fe_loss_function.ComputeJacobianMatrixAndResidualVector(current_control_vars, u)
-                res_norm = float(torch.linalg.norm(R.double()))
-                if res_norm != res_norm:
-                    raise ValueError("Residual norm contains NaN values.")
-                du = self.LinearSolve(K, R, u)
-                dnorm = float(torch.linalg.norm(du.double()))
-                hist["res_norm"].append(res_norm)
-                hist["delta_dofs_norm"].append(dnorm)
-                if res_norm < s["abs_tol"] or dnorm < s["rel_tol"] or i == s["maxiter"]:
-                    break
-                u = u + du
-        return u
+        flags = loss._flags() & L.ELEMENT_FIELD
+        if loss.scatter == "chunk":  # colour-ordered assembly, as ComputeJacobianMatrixAndResidualVector
+            flags |= L.SCATTER_COLOUR
+        try:
+            u, hist, steps = loss.ctx.newton(d, u, load_steps=s["load_incr"], max_iter=s["maxiter"],
+                                             abs_tol=s["abs_tol"], rel_tol=s["rel_tol"],
+                                             lin_tol=float(self.linear_solver_settings["tol"]),
+                                             lin_maxiter=int(self.linear_solver_settings["maxiter"]),
+                                             mode=self.mode, flags=flags)
+        except L.FemhotError as e:
+            if e.code == L.ERR_NONFINITE:
+                raise ValueError("Residual norm contains NaN values.") from e
+            raise
+        self.convergence_history = {
+            step + 1: {"res_norm": [float(v) for v in hist[step, :steps[step], 0]],
+                       "delta_dofs_norm": [float(v) for v in hist[step, :steps[step], 1]]}
+            for step in range(s["load_incr"])}
+        return u.to(_as_device_tensor(current_dofs_np).dtype)

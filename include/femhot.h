@@ -151,6 +151,25 @@ int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *c
                const double *b, double *x, double rel_tol, int maxiter, int *iters, double *rel_res,
                void *stream);
 
+/* Load-stepped Newton-Raphson solve of the BC-masked residual R(u; field) = 0, replacing
+ * FiniteElementNonLinearResidualBasedSolver.Solve (fol/solvers/fe_nonlinear_residual_based_solver.py:
+ * 76-170).  For load step s = 1..load_steps the Dirichlet dofs are set to (s / load_steps) x their
+ * value (fe_loss.py:91-92); each iteration assembles the FP64 CSR tangent and residual
+ * (femhot_tangent_csr), takes ||R||_2 (a NaN norm stops the solve with FEMHOT_ERR_NONFINITE, the
+ * reference's NaN guard, :118-132), solves K du = -R with femhot_pcg (mode FEMHOT_F64 or FEMHOT_F32;
+ * a solve that reaches lin_maxiter continues with its last iterate, as the reference's iterative
+ * solver does), and stops when ||R|| < abs_tol, ||du|| < rel_tol or after max_iter iterations,
+ * else u += du (:135-159).  All vectors stay on the device; the host waits twice per iteration
+ * (the two norms) besides the linear solver's own convergence checks.
+ * field: FP64 DEVICE, (n_nodes) or (n_elems) with FEMHOT_ELEMENT_FIELD.  u: FP64 DEVICE (n_dofs),
+ * initial guess in, solution out.  flags: FEMHOT_ELEMENT_FIELD | FEMHOT_SCATTER_COLOUR only.
+ * history: HOST (load_steps x max_iter x 2) doubles or NULL: (||R||, ||du||) per iteration, NaN
+ * after a load step's last iteration.  steps: HOST (load_steps) ints or NULL: iterations taken per
+ * load step.  Synchronises the stream. */
+int femhot_newton(femhot_ctx *ctx, int mode, const double *field, double *u, unsigned flags, int load_steps,
+                  int max_iter, double abs_tol, double rel_tol, double lin_tol, int lin_maxiter, double *history,
+                  int *steps, void *stream);
+
 /* Autotuner for the LDS-staged chunk path (FEMHOT_SCATTER_CHUNK): times elements-per-chunk x
  * workgroup size x grid shape (one workgroup per chunk, or a persistent grid of k workgroups per
  * CU) x samples per work item on these inputs, and the plain path beside them, and keeps the
