@@ -27,6 +27,8 @@
  *  - fol/solvers/fe_solver.py:60-78         LinearSolve (BiCGSTAB / direct on the assembled
  *                                           tangent) -> femhot_pcg (Jacobi-PCG, FP64 or FP32
  *                                           inner iterations with FP64 refinement)
+ *  - fol/solvers/fe_nonlinear_residual_based_solver.py:76-170  load-stepped Newton Solve
+ *                                           -> femhot_newton
  *  - fol/mesh_input_output/mesh.py:123-142  CheckAndOrientElements -> femhot_orient_elements
  */
 #ifndef FEMHOT_H
