@@ -181,9 +181,10 @@ template <class T> static int upload(T **dst, const std::vector<T> &src) {
 // Chunk plan for the LDS-staged scatter.  Elements are ordered by (segment, Morton code of the
 // centroid) and cut into runs of at most `epc` elements that never cross a segment; inside a chunk
 // the elements keep their original relative order (so per-element inputs/outputs are read and
-// written in contiguous runs).  Each chunk lists its nodes sorted by global id, flags the nodes
-// shared with other chunks, stores its elements' chunk-local connectivity (uint16) and the
-// coordinates of its nodes relative to its first node (FP32 and FP64).
+// written in contiguous runs).  Each chunk lists its nodes by number of incident corners in the
+// chunk (most first, ties by global id), flags the nodes shared with other chunks, stores its
+// elements' chunk-local connectivity (uint16) and the coordinates of its nodes relative to its
+// first node (FP32 and FP64).
 // Modeled HBM bytes of one chunk-path call for a candidate element order (chunks are runs of at
 // most epc elements within a segment; each run is sorted by element id here, as the plan keeps
 // it): 128-byte lines of the per-element field read and of its gradient written through the
@@ -286,7 +287,7 @@ int ensure_chunks(Ctx *c, int epc) {
     perm = c->chunk.perm_identity ? ident : morton;
   }
   long long i = 0;
-  std::vector<int> loc;
+  std::vector<int> loc, loc2, lid, icount, lorder, lrank;
   int max_nodes = 0, max_elems = 0;
   while (i < Ne) {
     long long j = i;
@@ -306,11 +307,30 @@ int ensure_chunks(Ctx *c, int epc) {
       set_error("ensure_chunks: too many element corners in a chunk");
       return FEMHOT_ERR_UNSUPPORTED;
     }
+    // Local node order: by number of incident corners in the chunk, most first (ties by node id).
+    // The node gather gives one lane per node and walks its incidence list, so a wave then walks
+    // lists of similar length instead of waiting on its longest one.
+    lid.resize((size_t)(j - i) * n);
+    icount.assign(loc.size(), 0);
     for (long long k = i; k < j; ++k)
       for (int a = 0; a < n; ++a) {
         const int g = c->h_conn[(size_t)perm[k] * n + a];
-        lconn[(size_t)k * n + a] = (uint16_t)(std::lower_bound(loc.begin(), loc.end(), g) - loc.begin());
+        const int q = (int)(std::lower_bound(loc.begin(), loc.end(), g) - loc.begin());
+        lid[(size_t)(k - i) * n + a] = q;
+        icount[q]++;
       }
+    lorder.resize(loc.size());
+    std::iota(lorder.begin(), lorder.end(), 0);
+    std::stable_sort(lorder.begin(), lorder.end(), [&](int x, int y) { return icount[x] > icount[y]; });
+    lrank.resize(loc.size());
+    loc2.resize(loc.size());
+    for (size_t q = 0; q < loc.size(); ++q) {
+      lrank[lorder[q]] = (int)q;
+      loc2[q] = loc[lorder[q]];
+    }
+    loc.swap(loc2);
+    for (long long k = i; k < j; ++k)
+      for (int a = 0; a < n; ++a) lconn[(size_t)k * n + a] = (uint16_t)lrank[lid[(size_t)(k - i) * n + a]];
     // incidence (counting sort by local node; entries of a node in element-position order)
     cnt.assign(loc.size() + 1, 0);
     for (long long k = i; k < j; ++k)
