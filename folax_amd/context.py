@@ -270,13 +270,16 @@ class FemhotContext:
                                         _stream()))
         return x, np.array(hist[:], dtype=np.float64).reshape(load_steps, max_iter, 2), list(steps)
 
-    def pcg(self, row_ptr, col_idx, vals, b, x0=None, tol=1e-10, maxiter=10000, mode="f64", raise_on_fail=True):
-        """Jacobi-PCG (FP64), or FP32 inner iterations + FP64 refinement (mode='mixed')."""
+    def pcg(self, row_ptr, col_idx, vals, b, x0=None, tol=1e-10, maxiter=10000, mode="f64", raise_on_fail=True,
+            context_pattern=False):
+        """Jacobi-PCG (FP64), or FP32 inner iterations + FP64 refinement (mode='mixed').  context_pattern:
+        row_ptr/col_idx are this context's pattern (`pattern()`), which lets the SpMV read one column
+        index per node block."""
         if vals.dtype != torch.float64 or b.dtype != torch.float64:
             raise TypeError("pcg expects the FP64 tangent and right-hand side")
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         it, rel = C.c_int(), C.c_double()
-        m = L.F64 if mode == "f64" else L.F32
+        m = (L.F64 if mode == "f64" else L.F32) | (L.PCG_CONTEXT_PATTERN if context_pattern else 0)
         rc = self._lib.femhot_pcg(self._h, m, _ptr(row_ptr), _ptr(col_idx), _ptr(vals), _ptr(b), _ptr(x), float(tol),
                                   int(maxiter), C.byref(it), C.byref(rel), _stream())
         if rc == L.ERR_NOT_CONVERGED and not raise_on_fail:

@@ -79,6 +79,70 @@ __global__ __launch_bounds__(256) void k_spmv(long long nrows, const long long *
   }
 }
 
+// y = A x on a node-blocked pattern: P dofs per node, and every dof row of node i lists the same
+// neighbours m, each as the P consecutive columns P*m .. P*m+P-1 (the pattern ensure_pattern builds).
+// LPN lanes per node; a lane takes every LPN-th neighbour and reads its column index once for the
+// whole P x P block (the other P*P-1 indices of the block are implied), then the P x values of the
+// neighbour as one contiguous block.  Index traffic drops from 4 to 4/(P*P) bytes per nonzero.
+// When part is set, part[block] = this workgroup's share of x.y (FP64).
+template <class T, int P, int LPN>
+__global__ __launch_bounds__(256) void k_spmv_node(long long nnodes, const long long *__restrict__ rp,
+                                                   const int *__restrict__ ci, const T *__restrict__ v,
+                                                   const T *__restrict__ x, T *__restrict__ y, double *part) {
+  constexpr int U = 4;
+  const long long node = ((long long)blockIdx.x * 256 + threadIdx.x) / LPN;
+  const int lane = (int)(threadIdx.x % LPN);
+  double d = 0.0;
+  if (node < nnodes) {
+    long long rb[P];
+#pragma unroll
+    for (int c = 0; c < P; ++c) rb[c] = rp[node * P + c];
+    const int deg = (int)((rp[node * P + 1] - rb[0]) / P);
+    double s[P];
+#pragma unroll
+    for (int c = 0; c < P; ++c) s[c] = 0.0;
+    for (int k0 = lane; k0 < deg; k0 += U * LPN) {
+      int col[U];
+      T vv[U][P][P];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * LPN;
+        const bool in = k < deg;
+        col[u] = in ? ci[rb[0] + (long long)P * k] : (int)(node * P);
+#pragma unroll
+        for (int c = 0; c < P; ++c)
+#pragma unroll
+          for (int e = 0; e < P; ++e) vv[u][c][e] = in ? v[rb[c] + (long long)P * k + e] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        T xv[P];
+#pragma unroll
+        for (int e = 0; e < P; ++e) xv[e] = x[col[u] + e];
+#pragma unroll
+        for (int c = 0; c < P; ++c)
+#pragma unroll
+          for (int e = 0; e < P; ++e) s[c] += (double)vv[u][c][e] * (double)xv[e];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < P; ++c)
+#pragma unroll
+      for (int off = LPN / 2; off >= 1; off >>= 1) s[c] += __shfl_xor(s[c], off, LPN);
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < P; ++c) {
+        y[node * P + c] = (T)s[c];
+        d += (double)x[node * P + c] * (double)(T)s[c];
+      }
+    }
+  }
+  if (part) {
+    const double t = block_sum256(d);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
 // out[0] = sum of part[0 .. n) in a fixed order (one workgroup of 1024; four independent
 // accumulators per thread keep four loads in flight).
 __global__ __launch_bounds__(1024) void k_sum_parts(const double *part, long long n, double *out) {
@@ -186,13 +250,50 @@ static int lanes_per_row(long long nnz, long long n) {
   return l;
 }
 
-static unsigned spmv_blocks(long long n, int lpr) { return (unsigned)((n * lpr + 255) / 256); }
+// How an SpMV is launched: row-wise CSR with `lanes` lanes per row, or node-blocked (P = 2 or 3 dofs
+// per node) with `lanes` lanes per node.  blocks = workgroups of 256 = number of x.y partials.
+struct SpmvPlan {
+  int P = 1;  // 1: row-wise CSR kernel
+  int lanes = 4;
+  unsigned blocks = 1;
+};
+
+static SpmvPlan spmv_plan(long long n, long long nnz, int P, bool node_blocked) {
+  SpmvPlan sp;
+  if (node_blocked && (P == 2 || P == 3) && n % P == 0) {
+    const long long nodes = n / P;
+    const double deg = nodes ? (double)nnz / (double)(P * P * nodes) : 1.0;  // neighbours per node
+    int l = 2;
+    while (l < 16 && 4.0 * l < deg) l *= 2;
+    sp.P = P;
+    sp.lanes = l;
+    sp.blocks = (unsigned)((nodes * l + 255) / 256);
+  } else {
+    sp.lanes = lanes_per_row(nnz, n);
+    sp.blocks = (unsigned)((n * sp.lanes + 255) / 256);
+  }
+  return sp;
+}
+
+template <class T, int P>
+static void spmv_node_launch(long long nodes, const long long *rp, const int *ci, const T *v, const T *x, T *y,
+                             double *part, const SpmvPlan &sp, hipStream_t s) {
+  const dim3 grid(sp.blocks);
+  switch (sp.lanes) {
+    case 2: hipLaunchKernelGGL((k_spmv_node<T, P, 2>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part); break;
+    case 4: hipLaunchKernelGGL((k_spmv_node<T, P, 4>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part); break;
+    case 8: hipLaunchKernelGGL((k_spmv_node<T, P, 8>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part); break;
+    default: hipLaunchKernelGGL((k_spmv_node<T, P, 16>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part); break;
+  }
+}
 
 template <class T>
 static void spmv_launch(long long n, const long long *rp, const int *ci, const T *v, const T *x, T *y, double *part,
-                        int lpr, hipStream_t s) {
-  const dim3 grid(spmv_blocks(n, lpr));
-  switch (lpr) {
+                        const SpmvPlan &sp, hipStream_t s) {
+  if (sp.P == 3) return spmv_node_launch<T, 3>(n / 3, rp, ci, v, x, y, part, sp, s);
+  if (sp.P == 2) return spmv_node_launch<T, 2>(n / 2, rp, ci, v, x, y, part, sp, s);
+  const dim3 grid(sp.blocks);
+  switch (sp.lanes) {
     case 4: hipLaunchKernelGGL((k_spmv<T, 4>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
     case 8: hipLaunchKernelGGL((k_spmv<T, 8>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
     case 16: hipLaunchKernelGGL((k_spmv<T, 16>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
@@ -217,11 +318,12 @@ static int host_sum_partials(Ctx *c, const double *part, hipStream_t s, double *
 template <class T>
 static int pcg_core(Ctx *c, long long n, const long long *rp, const int *ci, const T *v, const double *b, T *x,
                     T *r, T *p, T *q, T *dinv, double *sc, double *pap_part, double *pap, double bnorm, double tol,
-                    int maxiter, int check, int lpr, hipStream_t s, int *iters_out, double *relres, bool *converged) {
+                    int maxiter, int check, const SpmvPlan &sp, hipStream_t s, int *iters_out, double *relres,
+                    bool *converged) {
   const dim3 grid(NBR);
   double *rz[2] = {sc, sc + NBR}, *rrp = sc + 2 * NBR;
-  const unsigned nsp = spmv_blocks(n, lpr);
-  spmv_launch<T>(n, rp, ci, v, x, r, nullptr, lpr, s);  // r = A x
+  const unsigned nsp = sp.blocks;
+  spmv_launch<T>(n, rp, ci, v, x, r, nullptr, sp, s);  // r = A x
   hipLaunchKernelGGL((k_init<T>), grid, dim3(256), 0, s, n, b, r, p, dinv, rz[0], rrp);
   double rr = 0.0;
   int rc = host_sum_partials(c, rrp, s, &rr);
@@ -236,7 +338,7 @@ static int pcg_core(Ctx *c, long long n, const long long *rp, const int *ci, con
   int k = 0;
   while (k < maxiter) {
     const int par = k & 1;
-    spmv_launch<T>(n, rp, ci, v, p, q, pap_part, lpr, s);
+    spmv_launch<T>(n, rp, ci, v, p, q, pap_part, sp, s);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, s, pap_part, (long long)nsp, pap);
     hipLaunchKernelGGL((k_update_xr<T>), grid, dim3(256), 0, s, n, x, r, p, q, dinv, rz[par], pap, rz[1 - par], rrp);
     hipLaunchKernelGGL((k_update_p<T>), grid, dim3(256), 0, s, n, p, r, dinv, rz[1 - par], rz[par]);
@@ -293,7 +395,7 @@ extern "C" int femhot_spmv(femhot_ctx *h, const long long *row_ptr, const int *c
   Ctx *c = &h->c;
   int rc = ensure_pattern(c);
   if (rc) return rc;
-  spmv_launch<double>(c->Ndof, row_ptr, col_idx, vals, x, y, nullptr, lanes_per_row(c->nnz, c->Ndof),
+  spmv_launch<double>(c->Ndof, row_ptr, col_idx, vals, x, y, nullptr, spmv_plan(c->Ndof, c->nnz, 1, false),
                       (hipStream_t)stream);
   FH_HIP(hipGetLastError());
   return FEMHOT_OK;
@@ -303,7 +405,10 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
                           const double *b, double *x, double tol, int maxiter, int *iters, double *rel_res,
                           void *stream) {
   FH_CHECK_ARG(h && row_ptr && col_idx && vals && b && x, FEMHOT_ERR_ARG, "null argument");
-  FH_CHECK_ARG(mode == FEMHOT_F64 || mode == FEMHOT_F32, FEMHOT_ERR_ARG, "mode must be FEMHOT_F64 or FEMHOT_F32");
+  const bool node_blocked = (mode & FEMHOT_PCG_CONTEXT_PATTERN) != 0;
+  mode &= ~FEMHOT_PCG_CONTEXT_PATTERN;
+  FH_CHECK_ARG(mode == FEMHOT_F64 || mode == FEMHOT_F32, FEMHOT_ERR_ARG,
+               "mode must be FEMHOT_F64 or FEMHOT_F32 (| FEMHOT_PCG_CONTEXT_PATTERN)");
   FH_CHECK_ARG(tol > 0 && maxiter > 0, FEMHOT_ERR_ARG, "tol and maxiter must be positive");
   Ctx *c = &h->c;
   int rc = ensure_pattern(c);
@@ -314,8 +419,8 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
   const int check = 10;
   // workspace: reduction partials (5 x NBR doubles) + 5 FP64 vectors + (FP32 mode) 5 FP32 vectors + FP32 matrix
   const size_t vec64 = sizeof(double) * n, vec32 = sizeof(float) * n;
-  const int lpr = lanes_per_row(nnz, n);
-  const size_t nsp = spmv_blocks(n, lpr);
+  const SpmvPlan sp = spmv_plan(n, nnz, c->ndpn, node_blocked);
+  const size_t nsp = sp.blocks;
   size_t need = (4 * NBR + nsp + 8) * sizeof(double) + 1024 + 5 * vec64 + 1024;
   if (mode == FEMHOT_F32) need += 5 * vec32 + sizeof(float) * nnz + 1024;
   rc = ensure_workspace(c, need);
@@ -344,7 +449,7 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
   bool conv = false;
   if (mode == FEMHOT_F64) {
     rc = pcg_core<double>(c, n, row_ptr, col_idx, vals, b, x, r64, p64, q64, dinv64, sc, pap_part, pap, bnorm, tol,
-                          maxiter, check, lpr, s, &it, &rel, &conv);
+                          maxiter, check, sp, s, &it, &rel, &conv);
     if (rc) return rc;
   } else {
     float *v32 = (float *)take(sizeof(float) * nnz);
@@ -357,7 +462,7 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
     const double inner_tol = 1e-4;
     int outer = 0;
     while (true) {
-      spmv_launch<double>(n, row_ptr, col_idx, vals, x, ax64, nullptr, lpr, s);
+      spmv_launch<double>(n, row_ptr, col_idx, vals, x, ax64, nullptr, sp, s);
       hipLaunchKernelGGL(k_resid64, dim3(NBR), dim3(256), 0, s, n, b, ax64, r64, aux);
       double r2 = 0.0;
       rc = host_sum_partials(c, aux, s, &r2);
@@ -378,7 +483,7 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
       bool cin = false;
       // inner solve on the FP64 residual r64 (FP32 iterates, FP64 dot products)
       rc = pcg_core<float>(c, n, row_ptr, col_idx, v32, r64, x32, r32, p32, q32, dinv32, sc, pap_part, pap, bnorm * rel,
-                           std::max(inner_tol, 0.5 * tol / std::max(rel, 1e-300)), maxiter - it, check, lpr, s,
+                           std::max(inner_tol, 0.5 * tol / std::max(rel, 1e-300)), maxiter - it, check, sp, s,
                            &k_in, &rel_in, &cin);
       if (rc) return rc;
       it += std::max(k_in, 1);
@@ -484,7 +589,8 @@ extern "C" int femhot_newton(femhot_ctx *h, int mode, const double *field, doubl
       hipLaunchKernelGGL(k_newton_rhs, grid, dim3(256), 0, s, n, R, b, du);
       int lit = 0;
       double lrel = 0.0;
-      rc = femhot_pcg(h, mode, c->d_row_ptr, c->d_col_idx, vals, b, du, lin_tol, lin_maxiter, &lit, &lrel, stream);
+      rc = femhot_pcg(h, mode | FEMHOT_PCG_CONTEXT_PATTERN, c->d_row_ptr, c->d_col_idx, vals, b, du, lin_tol,
+                      lin_maxiter, &lit, &lrel, stream);
       if (rc && rc != FEMHOT_ERR_NOT_CONVERGED) return rc;  // an unconverged solve continues with its iterate
       hipLaunchKernelGGL(k_dot64, dim3(NBR), dim3(256), 0, s, n, du, du, part);
       double dd = 0.0;

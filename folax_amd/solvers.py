@@ -50,7 +50,8 @@ class FiniteElementSolver:
         x, it, rel, ok = ctx.pcg(tangent_matrix.row_ptr, tangent_matrix.col_idx,
                                  tangent_matrix.values.to(torch.float64).contiguous(), b,
                                  tol=float(self.linear_solver_settings["tol"]),
-                                 maxiter=int(self.linear_solver_settings["maxiter"]), mode=self.mode)
+                                 maxiter=int(self.linear_solver_settings["maxiter"]), mode=self.mode,
+                                 context_pattern=True)
         self.last_iterations, self.last_relative_residual = it, rel
         return x.to(residual_vector.dtype)
 

@@ -148,7 +148,11 @@ int femhot_energy_grad(femhot_ctx *ctx, int dtype, int batch, const void *field,
 /* Jacobi-preconditioned CG on the CSR tangent for K x = b (Dirichlet rows of b must be 0).
  * mode FEMHOT_F64: FP64 PCG.  mode FEMHOT_F32: FP32 matrix/vectors inside, FP64 dot products,
  * FP64 outer iterative refinement with the FP64 matrix.  x: in = initial guess, out = solution.
+ * mode | FEMHOT_PCG_CONTEXT_PATTERN declares that row_ptr/col_idx are this context's pattern
+ * (femhot_pattern): with 2 or 3 dofs per node the SpMV then reads one column index per node
+ * block instead of one per nonzero.  Without it any CSR matrix of n_dofs rows is accepted.
  * Returns FEMHOT_ERR_NOT_CONVERGED (x still written) when maxiter is reached. */
+#define FEMHOT_PCG_CONTEXT_PATTERN 0x100
 int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *col_idx, const double *vals,
                const double *b, double *x, double rel_tol, int maxiter, int *iters, double *rel_res,
                void *stream);

@@ -126,18 +126,20 @@ def test_hex64_tangent_null_space_symmetry_and_residual(hex64):
     assert trel(hex64["ctx"].spmv(rp, ci, vals, u), Ku) <= 1e-14
 
 
+@pytest.mark.parametrize("cp", [False, True])
 @pytest.mark.parametrize("mode", ["f64", "mixed"])
-def test_hex64_pcg(hex64, mode):
+def test_hex64_pcg(hex64, mode, cp):
     """K du = -R solved to 1e-10: the true residual (independent SpMV) meets the tolerance, Dirichlet
-    increments stay zero, and the FP64 and mixed-precision solutions agree."""
+    increments stay zero, and the FP64 and mixed-precision solutions agree; cp: the node-blocked SpMV
+    (FEMHOT_PCG_CONTEXT_PATTERN) against the row-wise one."""
     rp, ci, vals, R = hex64["rp"], hex64["ci"], hex64["vals"], hex64["R"]
     b = -R
-    x, it, rr, ok = hex64["ctx"].pcg(rp, ci, vals, b, tol=1e-10, maxiter=20000, mode=mode)
+    x, it, rr, ok = hex64["ctx"].pcg(rp, ci, vals, b, tol=1e-10, maxiter=20000, mode=mode, context_pattern=cp)
     assert ok and rr <= 1e-10
     true = float(torch.linalg.norm(b - csr_matvec(rp, ci, vals, x)) / torch.linalg.norm(b))
     assert true <= 2e-10, (mode, it, true)
     assert float(x[torch.as_tensor(hex64["didx"], device="cuda")].abs().max()) == 0.0
-    if mode == "f64":
+    if mode == "f64" and not cp:
         hex64["x64"] = x
     else:
         assert "x64" in hex64

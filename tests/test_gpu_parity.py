@@ -189,9 +189,11 @@ def test_element_field_and_segments(dt):
     assert rel(en.cpu().numpy().ravel(), seg_ref) <= TOL[dt]
 
 
+@pytest.mark.parametrize("cp", [False, True])
 @pytest.mark.parametrize("mode", ["f64", "mixed"])
-@pytest.mark.parametrize("kind,etype", [("thermal", "quad"), ("mechanical", "hexahedron"), ("mechanical", "tetra")])
-def test_pcg_against_direct_solve(kind, etype, mode):
+@pytest.mark.parametrize("kind,etype", [("thermal", "quad"), ("mechanical", "quad"), ("mechanical", "hexahedron"),
+                                        ("mechanical", "tetra")])
+def test_pcg_against_direct_solve(kind, etype, mode, cp):
     case = make_case(kind, etype, n=5, seed=8)
     ph = case["ph"]
     ub = fe.apply_dirichlet(case["u"], case["didx"], case["dval"])
@@ -201,7 +203,7 @@ def test_pcg_against_direct_solve(kind, etype, mode):
     ctx = ctx_for(case)
     rp, ci = ctx.pattern()
     vals, Rg = ctx.tangent_csr(dev(case["field"], "f64"), dev(ub, "f64"))
-    x, it, relres, ok = ctx.pcg(rp, ci, vals, -Rg, tol=1e-11, maxiter=5000, mode=mode)
+    x, it, relres, ok = ctx.pcg(rp, ci, vals, -Rg, tol=1e-11, maxiter=5000, mode=mode, context_pattern=cp)
     torch.cuda.synchronize()
     assert ok and relres <= 1e-11
     assert rel(x.cpu().numpy(), du_ref) <= 1e-9
