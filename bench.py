@@ -84,6 +84,29 @@ def cpu_baseline(W, budget_s=10.0):
                       f"{t:.1f} s"}, out
 
 
+def solver_profile(iters=200):
+    """HEX8 64^3 linear elasticity (274,625 nodes, 823,875 dofs), E in {1, 10} from a seeded Gaussian random
+    field (correlation length 8 voxels), clamped at z = 0 with 1 % stretch at z = 1: FP64 tangent + residual
+    (colour-ordered assembly), then `iters` FP64 Jacobi-PCG iterations with the node-blocked SpMV."""
+    import torch
+    from folax_amd import loss_functions as lf
+    from folax_amd.controls import grf_two_phase
+    from folax_amd.mesh import create_3D_box_mesh
+    m = create_3D_box_mesh(64, 64, 64)
+    bc = {"Ux": {"bottom": 0.0}, "Uy": {"bottom": 0.0}, "Uz": {"bottom": 0.0, "top": 0.01}}
+    loss = lf.MechanicalLoss3DHexa("c2", {"dirichlet_bc_dict": bc, "num_gp": 2,
+                                          "material_dict": {"young_modulus": 1.0, "poisson_ratio": 0.3}}, m)
+    loss.Initialize()
+    dev = torch.device("cuda")
+    E = torch.as_tensor(grf_two_phase((65, 65, 65), 8.0, 2001), dtype=torch.float64, device=dev)
+    u = loss.ApplyDirichletBCOnDofVector(torch.zeros(3 * m.GetNumberOfNodes(), dtype=torch.float64, device=dev))
+    K, R = loss.ComputeJacobianMatrixAndResidualVector(E, u)
+    x, it, rr, ok = loss.ctx.pcg(K.row_ptr, K.col_idx, K.values, (-R).contiguous(), tol=1e-14, maxiter=iters,
+                                 raise_on_fail=False, context_pattern=True)
+    torch.cuda.synchronize()
+    log(f"solver profile: HEX8 64^3 tangent nnz {K.values.numel()}, {it} FP64 PCG iterations, rel residual {rr:.3e}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +118,9 @@ def main():
     ap.add_argument("--epc", type=int, default=0, help="fixed elements per chunk (skips the autotuner)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--grid", type=int, default=0, help="workgroups per CU (0: one per chunk)")
+    ap.add_argument("--solver-profile", action="store_true",
+                    help="short run for rocprofv3 of the linear-solver kernels instead: HEX8 64^3 two-phase "
+                         "elastic tangent (colour-ordered FP64 assembly) + 200 FP64 PCG iterations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build has no multi-GPU path)",
@@ -107,6 +133,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.solver_profile:
+        return solver_profile()
     t0 = time.time()
     W = build_workload()
     Ne, Nn = W["conn"].shape[0], W["coords"].shape[0]
