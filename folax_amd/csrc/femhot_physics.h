@@ -91,6 +91,83 @@ __device__ __forceinline__ void add_nodal_force(const R (&S)[D][D], const R (&DN
     }
 }
 
+// Gradient G[i][j] = sum_a v_(a,i) dN_a/dx_j.  For TET4 dN_0 = -(dN_1 + dN_2 + dN_3), so G is formed
+// from the differences v_a - v_0 of nodes 1..3 and node 0's gradient is never needed.
+template <class R, int ET, int N, int D>
+__device__ __forceinline__ void elem_grad(const R (&v)[N * D], const R (&DN)[N][D], R (&G)[D][D]) {
+  if constexpr (ET == TET4) {
+    R dv[3][D];
+#pragma unroll
+    for (int a = 1; a < 4; ++a)
+#pragma unroll
+      for (int i = 0; i < D; ++i) dv[a - 1][i] = v[a * D + i] - v[i];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) G[i][j] = (dv[0][i] * DN[1][j] + dv[1][i] * DN[2][j]) + dv[2][i] * DN[3][j];
+  } else {
+    nodal_grad<R, N, D>(v, DN, G);
+  }
+}
+
+// r_(a,i) += sum_j Sf_ij dN_a,j for a stress already multiplied by its Gauss factor.  For TET4 node 0
+// receives minus the sum of nodes 1..3 (dN_0 = -(dN_1 + dN_2 + dN_3)).
+template <class R, int ET, int N, int D>
+__device__ __forceinline__ void add_scaled_nodal_force(const R (&Sf)[D][D], const R (&DN)[N][D], R (&r)[N * D]) {
+  if constexpr (ET == TET4) {
+    R t[3][D];
+#pragma unroll
+    for (int a = 1; a < 4; ++a)
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        R s = Sf[i][0] * DN[a][0];
+#pragma unroll
+        for (int j = 1; j < D; ++j) s += Sf[i][j] * DN[a][j];
+        t[a - 1][i] = s;
+        r[a * D + i] += s;
+      }
+#pragma unroll
+    for (int i = 0; i < D; ++i) r[i] -= (t[0][i] + t[1][i]) + t[2][i];
+  } else {
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        R s = Sf[i][0] * DN[a][0];
+#pragma unroll
+        for (int j = 1; j < D; ++j) s += Sf[i][j] * DN[a][j];
+        r[a * D + i] += s;
+      }
+  }
+}
+
+// f * S for a symmetric S (each distinct entry scaled once).
+template <class R, int D> __device__ __forceinline__ void scale_sym(const R (&S)[D][D], R f, R (&Sf)[D][D]) {
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = i; j < D; ++j) Sf[i][j] = Sf[j][i] = f * S[i][j];
+}
+
+template <class R, int D> __device__ __forceinline__ void scale_full(const R (&S)[D][D], R f, R (&Sf)[D][D]) {
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Sf[i][j] = f * S[i][j];
+}
+
+// A : S for a symmetric S: diagonal terms plus the symmetrised off-diagonal pairs.
+template <class R, int D> __device__ __forceinline__ R ddot_sym(const R (&A)[D][D], const R (&S)[D][D]) {
+  R s = A[0][0] * S[0][0];
+#pragma unroll
+  for (int i = 1; i < D; ++i) s += A[i][i] * S[i][i];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = i + 1; j < D; ++j) s += (A[i][j] + A[j][i]) * S[i][j];
+  return s;
+}
+
 template <class R, int N, int D>
 __device__ __forceinline__ void nh_grad_u(const R (&u)[N * D], const R (&DN)[N][D], R (&H)[D][D]) {
   nodal_grad<R, N, D>(u, DN, H);
@@ -134,10 +211,11 @@ __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const P
         r[a] += f * s - b * gp.Nv[a];
       }
     } else if constexpr (PH == MECH) {
-      R Gu[D][D], S[D][D];
-      nodal_grad<R, N, D>(el.u, gp.DN, Gu);
+      R Gu[D][D], S[D][D], Sf[D][D];
+      elem_grad<R, ET, N, D>(el.u, gp.DN, Gu);
       lin_stress<R, D>(Gu, pp, S);
-      add_nodal_force<R, N, D>(S, gp.DN, gp.wdet * eg, r);
+      scale_sym<R, D>(S, gp.wdet * eg, Sf);
+      add_scaled_nodal_force<R, ET, N, D>(Sf, gp.DN, r);
       if (pp.has_body) {
 #pragma unroll
         for (int a = 0; a < N; ++a)
@@ -146,7 +224,7 @@ __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const P
       }
     } else {
       R H[D][D];
-      nodal_grad<R, N, D>(el.u, gp.DN, H);
+      elem_grad<R, ET, N, D>(el.u, gp.DN, H);
       NHState<R, D> st;
       nh_state<R, D>(H, R(pp.kap1), R(pp.mu1), st);
       // P = F S ; r_(a,i) += wdet e sum_j P_ij dN_a,j
@@ -160,7 +238,9 @@ __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const P
           for (int k = 0; k < D; ++k) s += st.F[i][k] * st.S[k][j];
           Pm[i][j] = s;
         }
-      add_nodal_force<R, N, D>(Pm, gp.DN, gp.wdet * eg, r);
+      R Pf[D][D];
+      scale_full<R, D>(Pm, gp.wdet * eg, Pf);
+      add_scaled_nodal_force<R, ET, N, D>(Pf, gp.DN, r);
       if (pp.has_body) {
 #pragma unroll
         for (int a = 0; a < N; ++a)
@@ -232,17 +312,18 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
       }
       q = wGT * fac;
     } else if constexpr (PH == MECH) {
-      R Gu[D][D], Gw[D][D], Su[D][D], Sw[D][D];
-      nodal_grad<R, N, D>(el.u, gp.DN, Gu);
-      nodal_grad<R, N, D>(w, gp.DN, Gw);
+      R Gu[D][D], Gw[D][D], Su[D][D], Sw[D][D], Swf[D][D];
+      elem_grad<R, ET, N, D>(el.u, gp.DN, Gu);
+      elem_grad<R, ET, N, D>(w, gp.DN, Gw);
       lin_stress<R, D>(Gu, pp, Su);
       lin_stress<R, D>(Gw, pp, Sw);
-      add_nodal_force<R, N, D>(Sw, gp.DN, gp.wdet * eg, gu);
-      q = gp.wdet * ddot<R, D>(Gw, Su);
+      scale_sym<R, D>(Sw, gp.wdet * eg, Swf);
+      add_scaled_nodal_force<R, ET, N, D>(Swf, gp.DN, gu);
+      q = gp.wdet * ddot_sym<R, D>(Gw, Su);
     } else {
       R H[D][D], dF[D][D];
-      nodal_grad<R, N, D>(el.u, gp.DN, H);
-      nodal_grad<R, N, D>(w, gp.DN, dF);
+      elem_grad<R, ET, N, D>(el.u, gp.DN, H);
+      elem_grad<R, ET, N, D>(w, gp.DN, dF);
       NHState<R, D> st;
       nh_state<R, D>(H, R(pp.kap1), R(pp.mu1), st);
       // X = sym(F^T dF)
@@ -269,8 +350,10 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
           for (int k = 0; k < D; ++k) s += st.F[i][k] * dS[k][j] + dF[i][k] * st.S[k][j];
           M[i][j] = s;
         }
-      add_nodal_force<R, N, D>(M, gp.DN, gp.wdet * eg, gu);
-      q = gp.wdet * ddot<R, D>(X, st.S);
+      R Mf[D][D];
+      scale_full<R, D>(M, gp.wdet * eg, Mf);
+      add_scaled_nodal_force<R, ET, N, D>(Mf, gp.DN, gu);
+      q = gp.wdet * ddot_sym<R, D>(X, st.S);
     }
     if (el.elemental)
       gd[0] += q;
@@ -326,11 +409,12 @@ __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const Phys
       q = R(0.5) * gp.wdet * g2;
       pi += eg * q - b * Tg;
     } else if constexpr (PH == MECH) {
-      R Gu[D][D], S[D][D];
-      nodal_grad<R, N, D>(el.u, gp.DN, Gu);
+      R Gu[D][D], S[D][D], Sf[D][D];
+      elem_grad<R, ET, N, D>(el.u, gp.DN, Gu);
       lin_stress<R, D>(Gu, pp, S);
-      add_nodal_force<R, N, D>(S, gp.DN, gp.wdet * eg, r);
-      q = R(0.5) * gp.wdet * ddot<R, D>(Gu, S);
+      scale_sym<R, D>(S, gp.wdet * eg, Sf);
+      add_scaled_nodal_force<R, ET, N, D>(Sf, gp.DN, r);
+      q = R(0.5) * gp.wdet * ddot_sym<R, D>(Gu, S);
       R work = R(-0.0);
       if (pp.has_body) {
 #pragma unroll
@@ -345,7 +429,7 @@ __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const Phys
       pi += eg * q - work;
     } else {
       R H[D][D];
-      nodal_grad<R, N, D>(el.u, gp.DN, H);
+      elem_grad<R, ET, N, D>(el.u, gp.DN, H);
       NHState<R, D> st;
       nh_state<R, D>(H, R(pp.kap1), R(pp.mu1), st);
       R Pm[D][D];
@@ -358,7 +442,9 @@ __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const Phys
           for (int k = 0; k < D; ++k) s += st.F[i][k] * st.S[k][j];
           Pm[i][j] = s;
         }
-      add_nodal_force<R, N, D>(Pm, gp.DN, gp.wdet * eg, r);
+      R Pf[D][D];
+      scale_full<R, D>(Pm, gp.wdet * eg, Pf);
+      add_scaled_nodal_force<R, ET, N, D>(Pf, gp.DN, r);
       q = gp.wdet * st.psi;
       R work = R(-0.0);
       if (pp.has_body) {
