@@ -164,7 +164,6 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   const bool elemental = (flags & FEMHOT_ELEMENT_FIELD) != 0;
   const int P = c->ndpn;
   const size_t rs = dtype == FEMHOT_F32 ? sizeof(float) : sizeof(double);
-  const int PP = P == 3 ? 4 : P;  // padded per-node record (see k_chunk)
   const bool nodal_gd = od && !elemental && mode != MODE_RES;
   const size_t cap4 = ((size_t)p.max_nodes + 3) & ~(size_t)3;
   if (cfg.dma && dtype != FEMHOT_F32) {
@@ -175,7 +174,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
       cfg.dma ? (size_t)pipe_layout(p.max_nodes, p.max_elems, c->n, P, mode, elemental, ou != nullptr, nodal_gd,
                                     p.perm_identity)
                     .total
-              : cap4 * (4 + PP + (mode == MODE_VJP ? PP : 0) + (elemental ? 0 : 1)) * rs +
+              : cap4 * (3 + P + (mode == MODE_VJP ? P : 0) + (elemental ? 0 : 1)) * rs +
                     (size_t)p.max_elems * c->n * (P + (nodal_gd ? 1 : 0)) * rs + (size_t)p.max_elems * c->n * 2 + 16;
   if (shm > lds_limit()) {
     set_error("run_chunk: chunk too large for LDS; use fewer elements per chunk");

@@ -68,9 +68,9 @@ template <int N> __device__ __forceinline__ void load_lconn(const uint16_t *p, i
   }
 }
 
-// Per-node LDS records are padded to 4 values when P == 3 so they are read / written with one
-// 16-byte ds access; PP is the padded stride.
-template <int P> struct Pad { static constexpr int PP = (P == 3) ? 4 : P; };
+// Per-node LDS records are unpadded (3 coordinates, P solution values): the lanes of a wave read
+// records of randomly indexed nodes, and a 12-byte stride spreads them over every bank where a
+// 16-byte one would map node l and node l + 16 to the same banks.
 
 // An element's K per-corner results (K a multiple of 16 bytes) written with 16-byte LDS stores.
 // Consecutive lanes hold consecutive elements, so with K * sizeof(R) = 48 / 96 bytes each group
@@ -89,9 +89,9 @@ template <class R, int K> __device__ __forceinline__ void store_corner_results(R
 
 // LDS layout of k_chunk (in units of R unless noted; cap4 = max chunk nodes rounded up to 4,
 // emax = max chunk elements):
-//   X   [cap4 * 4]      chunk-relative coordinates (x, y, z, 0)
-//   U   [cap4 * PP]     nodal solution
-//   W   [cap4 * PP]     cotangent (VJP only)
+//   X   [cap4 * 3]      chunk-relative coordinates (x, y, z)
+//   U   [cap4 * P]      nodal solution
+//   W   [cap4 * P]      cotangent (VJP only)
 //   FD  [cap4]          nodal field (nodal fields only)
 //   ER  [emax * N * P]  per element corner ([position][corner][component]): residual / g_u.  An
 //                       incidence entry q = position * N + corner addresses its P values at q * P.
@@ -100,7 +100,7 @@ template <class R, int K> __device__ __forceinline__ void store_corner_results(R
 template <class R, int ET, int PH, int NGP, int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   using E = Elem<R, ET, PH>;
-  constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL, PP = Pad<P>::PP;
+  constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL;
   // nodes / elements per lane per round of loads: enough that one round covers 512 nodes and
   // 1024 elements (a 1024-element chunk's nodes and elements) without idle registers at 512 threads
   constexpr int KN = BLOCK >= 512 ? 1 : (BLOCK == 256 ? 2 : 4);
@@ -116,9 +116,9 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   const bool nodal_gd = want_gd && !elemental;
   const int cap4 = (A.cap + 3) & ~3, emax = A.emax;
   R *X = reinterpret_cast<R *>(smem_raw);
-  R *U = X + 4 * cap4;
-  R *W = U + PP * cap4;
-  R *FD = W + (MODE == MODE_VJP ? PP * cap4 : 0);
+  R *U = X + 3 * cap4;
+  R *W = U + P * cap4;
+  R *FD = W + (MODE == MODE_VJP ? P * cap4 : 0);
   R *ER = FD + (elemental ? 0 : cap4);
   R *ERD = ER + (long)emax * N * P;
   uint16_t *INC = reinterpret_cast<uint16_t *>(ERD + (nodal_gd ? (long)emax * N : 0));
@@ -147,10 +147,16 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       const R *xyz = reinterpret_cast<const R *>(A.xyz) + 4LL * n0;
       for (int i = tid; i < nl; i += BLOCK) {
         if constexpr (sizeof(R) == 4) {
-          reinterpret_cast<float4 *>(X)[i] = reinterpret_cast<const float4 *>(xyz)[i];
+          const float4 v = reinterpret_cast<const float4 *>(xyz)[i];
+          X[3 * i] = v.x;
+          X[3 * i + 1] = v.y;
+          X[3 * i + 2] = v.z;
         } else {
-          reinterpret_cast<double2 *>(X)[2 * i] = reinterpret_cast<const double2 *>(xyz)[2 * i];
-          reinterpret_cast<double2 *>(X)[2 * i + 1] = reinterpret_cast<const double2 *>(xyz)[2 * i + 1];
+          const double2 v0 = reinterpret_cast<const double2 *>(xyz)[2 * i];
+          const double2 v1 = reinterpret_cast<const double2 *>(xyz)[2 * i + 1];
+          X[3 * i] = v0.x;
+          X[3 * i + 1] = v0.y;
+          X[3 * i + 2] = v1.x;
         }
       }
       if (want_ou || nodal_gd) {  // one element's worth (N entries, 8 or 16 bytes) per access
@@ -171,9 +177,9 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
           const int i = base + k * BLOCK + tid;
           if (i < nl) {
 #pragma unroll
-            for (int c = 0; c < PP; ++c) {
-              U[i * PP + c] = c < P ? uv[k][c < P ? c : 0] : R(0);
-              if constexpr (MODE == MODE_VJP) W[i * PP + c] = c < P ? wv[k][c < P ? c : 0] : R(0);
+            for (int c = 0; c < P; ++c) {
+              U[i * P + c] = uv[k][c];
+              if constexpr (MODE == MODE_VJP) W[i * P + c] = wv[k][c];
             }
             if (!elemental) FD[i] = fv[k];
           }
@@ -248,20 +254,9 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
           E el;
           R xr[N][3];
 #pragma unroll
-          for (int a = 0; a < N; ++a) {
-            if constexpr (sizeof(R) == 4) {
-              const float4 v = reinterpret_cast<const float4 *>(X)[l[k][a]];
-              xr[a][0] = v.x;
-              xr[a][1] = v.y;
-              xr[a][2] = v.z;
-            } else {
-              const double2 v0 = reinterpret_cast<const double2 *>(X)[2 * l[k][a]];
-              const double2 v1 = reinterpret_cast<const double2 *>(X)[2 * l[k][a] + 1];
-              xr[a][0] = v0.x;
-              xr[a][1] = v0.y;
-              xr[a][2] = v1.x;
-            }
-          }
+          for (int a = 0; a < N; ++a)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) xr[a][i] = X[3 * l[k][a] + i];
 #pragma unroll
           for (int a = 0; a < N; ++a)
 #pragma unroll
@@ -277,17 +272,9 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             for (int a = 0; a < N; ++a) el.d[a] = FD[l[k][a]];
           }
 #pragma unroll
-          for (int a = 0; a < N; ++a) {
-            if constexpr (P == 3 && sizeof(R) == 4) {
-              const float4 v = reinterpret_cast<const float4 *>(U)[l[k][a]];
-              el.u[a * 3 + 0] = v.x;
-              el.u[a * 3 + 1] = v.y;
-              el.u[a * 3 + 2] = v.z;
-            } else {
+          for (int a = 0; a < N; ++a)
 #pragma unroll
-              for (int c = 0; c < P; ++c) el.u[a * P + c] = U[l[k][a] * PP + c];
-            }
-          }
+            for (int c = 0; c < P; ++c) el.u[a * P + c] = U[l[k][a] * P + c];
           R ro[NL], gd[N];
           if constexpr (MODE == MODE_RES) {
             R energy;
@@ -298,7 +285,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
 #pragma unroll
             for (int a = 0; a < N; ++a)
 #pragma unroll
-              for (int c = 0; c < P; ++c) w[a * P + c] = W[l[k][a] * PP + c];
+              for (int c = 0; c < P; ++c) w[a * P + c] = W[l[k][a] * P + c];
             elem_vjp<R, ET, PH, NGP>(el, w, A.pp, picard, ro, gd);
           } else {
             R pi;
