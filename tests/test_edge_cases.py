@@ -84,6 +84,23 @@ def test_ctx_create_rejects_bad_meshes(case, code):
     assert len(L.lib().femhot_last_error()) > 0
 
 
+def test_solver_and_tuning_entry_points_validate_on_the_host():
+    """The solver, Newton and tuning entry points reject a null context, and the Newton driver its
+    settings, before any device work (no GPU needed)."""
+    from folax_amd import _lib as L
+    lib = L.lib()
+    dbl = C.c_double()
+    it = C.c_int()
+    hist = (C.c_double * 4)()
+    steps = (C.c_int * 2)()
+    assert lib.femhot_pcg(None, L.F64, None, None, None, None, None, 1e-8, 10, C.byref(it), C.byref(dbl),
+                          None) == L.ERR_ARG
+    assert lib.femhot_newton(None, L.F64, None, None, 0, 1, 1, 1e-8, 1e-8, 1e-10, 10, hist, steps, None) == L.ERR_ARG
+    assert lib.femhot_set_chunk_config(None, 0, L.F32, 1, 0, 256, 256, 0) == L.ERR_ARG
+    assert lib.femhot_residual_shape_vjp(None, L.F64, None, None, None, None, 0, None) == L.ERR_ARG
+    assert len(lib.femhot_last_error()) > 0
+
+
 # ------------------------------------------------------------------------------------------------
 # GPU: degenerate but legal meshes
 # ------------------------------------------------------------------------------------------------
