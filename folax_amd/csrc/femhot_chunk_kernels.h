@@ -346,10 +346,15 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
               R s[P];
 #pragma unroll
               for (int c = 0; c < P; ++c) s[c] = R(-0.0);
+              // the next entry is read while this one's corner values load (the list is non-empty
+              // for every chunk node, so index je - 1 is always in the list)
+              int q = INC[js[k]];
               for (int j = js[k]; j < je[k]; ++j) {
-                const R *e = ER + INC[j] * P;
+                const int qn = INC[min(j + 1, je[k] - 1)];
+                const R *e = ER + q * P;
 #pragma unroll
                 for (int c = 0; c < P; ++c) s[c] += e[c];
+                q = qn;
               }
 #pragma unroll
               for (int c = 0; c < P; ++c) {
