@@ -123,7 +123,7 @@ def main():
                          "elastic tangent (colour-ordered FP64 assembly) + 200 FP64 PCG iterations")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (this build has no multi-GPU path)",
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the library has no multi-GPU path)",
               file=sys.stderr)
         sys.exit(2)
     import torch
