@@ -105,6 +105,10 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   // 1024 elements (a 1024-element chunk's nodes and elements) without idle registers at 512 threads
   constexpr int KN = BLOCK >= 512 ? 1 : (BLOCK == 256 ? 2 : 4);
   constexpr int KE = (ET != HEX8 && NGP <= 2) ? (BLOCK >= 512 ? 2 : 4) : 1;
+  // Light element routines (several elements per lane per round) keep the first round of node
+  // metadata and element connectivity in registers from the plan load on; heavy ones (HEX8,
+  // high-order rules) need those registers for the element arithmetic and reload them where used.
+  constexpr bool PF = KE > 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[BLOCK / 64];
   const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
@@ -228,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       for (int base = e0; base < e1; base += KE * BLOCK) {
         int l[KE][N], ee[KE];
         R dev[KE];
-        if (base == e0) {
+        if (PF && base == e0) {
 #pragma unroll
           for (int k = 0; k < KE; ++k) {
             unpack_lconn<N>(lw0[k], l[k]);
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       if (want_ou || nodal_gd) {
         for (int base = 0; base < nl; base += KN * BLOCK) {
           int g[KN], sl[KN], js[KN], je[KN];
-          if (base == 0) {
+          if (PF && base == 0) {
 #pragma unroll
             for (int k = 0; k < KN; ++k) g[k] = g0[k], sl[k] = sl0[k], js[k] = js0[k], je[k] = je0[k];
           } else {

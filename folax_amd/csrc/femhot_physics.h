@@ -156,6 +156,19 @@ template <class R, int D> __device__ __forceinline__ void scale_full(const R (&S
     for (int j = 0; j < D; ++j) Sf[i][j] = f * S[i][j];
 }
 
+// r_(a,i) += f * sum_j M_ij dN_a,j for a general (non-symmetric) M: scaled once and node 0 taken from
+// the others for TET4; elsewhere the factor is applied per entry (no scaled copy of M in registers).
+template <class R, int ET, int N, int D>
+__device__ __forceinline__ void add_nodal_force_f(const R (&M)[D][D], const R (&DN)[N][D], R f, R (&r)[N * D]) {
+  if constexpr (ET == TET4) {
+    R Mf[D][D];
+    scale_full<R, D>(M, f, Mf);
+    add_scaled_nodal_force<R, ET, N, D>(Mf, DN, r);
+  } else {
+    add_nodal_force<R, N, D>(M, DN, f, r);
+  }
+}
+
 // A : S for a symmetric S: diagonal terms plus the symmetrised off-diagonal pairs.
 template <class R, int D> __device__ __forceinline__ R ddot_sym(const R (&A)[D][D], const R (&S)[D][D]) {
   R s = A[0][0] * S[0][0];
@@ -238,9 +251,7 @@ __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const P
           for (int k = 0; k < D; ++k) s += st.F[i][k] * st.S[k][j];
           Pm[i][j] = s;
         }
-      R Pf[D][D];
-      scale_full<R, D>(Pm, gp.wdet * eg, Pf);
-      add_scaled_nodal_force<R, ET, N, D>(Pf, gp.DN, r);
+      add_nodal_force_f<R, ET, N, D>(Pm, gp.DN, gp.wdet * eg, r);
       if (pp.has_body) {
 #pragma unroll
         for (int a = 0; a < N; ++a)
@@ -350,9 +361,7 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
           for (int k = 0; k < D; ++k) s += st.F[i][k] * dS[k][j] + dF[i][k] * st.S[k][j];
           M[i][j] = s;
         }
-      R Mf[D][D];
-      scale_full<R, D>(M, gp.wdet * eg, Mf);
-      add_scaled_nodal_force<R, ET, N, D>(Mf, gp.DN, gu);
+      add_nodal_force_f<R, ET, N, D>(M, gp.DN, gp.wdet * eg, gu);
       q = gp.wdet * ddot_sym<R, D>(X, st.S);
     }
     if (el.elemental)
@@ -442,9 +451,7 @@ __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const Phys
           for (int k = 0; k < D; ++k) s += st.F[i][k] * st.S[k][j];
           Pm[i][j] = s;
         }
-      R Pf[D][D];
-      scale_full<R, D>(Pm, gp.wdet * eg, Pf);
-      add_scaled_nodal_force<R, ET, N, D>(Pf, gp.DN, r);
+      add_nodal_force_f<R, ET, N, D>(Pm, gp.DN, gp.wdet * eg, r);
       q = gp.wdet * st.psi;
       R work = R(-0.0);
       if (pp.has_body) {
