@@ -106,19 +106,22 @@ def test_newton_nan_guard_and_mixed_precision_solve():
         s.Solve(t64(kb), np.zeros(m.GetNumberOfNodes()))
 
 
-def test_newton_neohooke_quadratic_convergence():
-    """Consistent-tangent Newton on a compressible Neo-Hookean HEX8 block under 5 % stretch: same solution
-    as the oracle's Newton iteration, and ||R|| falls quadratically near the solution."""
+@pytest.mark.parametrize("elemental", [False, True])
+def test_newton_neohooke_quadratic_convergence(elemental):
+    """Consistent-tangent Newton on a compressible Neo-Hookean HEX8 block under 5 % stretch, with a nodal or a
+    per-element field: same solution as the oracle's Newton iteration, and ||R|| falls quadratically near
+    the solution."""
     from folax_amd import _lib as L
     from tests.cases import femhot_context
-    case = make_case("neohooke", "hexahedron", n=4, seed=9)
+    case = make_case("neohooke", "hexahedron", n=4, seed=9, elemental=elemental)
     # stretch the top face by 5 %: large enough that the NH response is visibly nonlinear
     idx, val = case["didx"], np.where(case["dval"] != 0.0, 0.05, 0.0)
     case["dval"] = val
     ctx = femhot_context(case)
     Nn = case["coords"].shape[0]
     u, hist, steps = ctx.newton(t64(case["field"]), t64(np.zeros(3 * Nn)), load_steps=2, max_iter=12, abs_tol=1e-11,
-                                rel_tol=1e-12, lin_tol=1e-13, lin_maxiter=20000)
+                                rel_tol=1e-12, lin_tol=1e-13, lin_maxiter=20000,
+                                flags=L.ELEMENT_FIELD if elemental else 0)
     ur, href = oracle_newton(case["ph"], case["coords"], case["conn"], case["field"], idx, val, np.zeros(3 * Nn), 2, 12,
                              1e-11, 1e-12)
     assert rel(u.cpu().numpy(), ur) <= 1e-9
