@@ -108,7 +108,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   // Light element routines (several elements per lane per round) keep the first round of node
   // metadata and element connectivity in registers from the plan load on; heavy ones (HEX8,
   // high-order rules) need those registers for the element arithmetic and reload them where used.
-  constexpr bool PF = KE > 1;
+  constexpr bool PF = KE > 1 && BLOCK >= 512;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[BLOCK / 64];
   const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
