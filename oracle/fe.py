@@ -604,7 +604,8 @@ def assemble(ph: Physics, coords, conn, field, u, dirichlet_idx, transpose=False
 
 def csr_pattern(conn, ndpn, n_nodes):
     """Sparsity of the assembled tangent: scipy csr_array semantics on the reference COO
-    (duplicates summed, columns sorted, explicit zeros kept).  Returns (indptr, indices) int64."""
+    (duplicates summed, columns sorted, explicit zeros kept).  Returns (indptr, indices) int64.
+    Pinned by a brute-force node-adjacency construction and the closed-form nnz (3N-2)^d ndpn^2."""
     dofs = element_dofs(conn, ndpn)
     nl = dofs.shape[1]
     rows = np.repeat(dofs, nl, axis=1).reshape(-1)
@@ -641,7 +642,8 @@ def scatter_nodal(conn, vals, n_nodes, ndpn):
 
 
 def global_residual_vjp(ph, coords, conn, field, u, w, picard=False):
-    """Assembled residual VJP: g_u = (dR/du)^T w, g_d = (dR/dd)^T w (no BC masking)."""
+    """Assembled residual VJP: g_u = (dR/du)^T w, g_d = (dR/dd)^T w (no BC masking).  Pinned through
+    residual_vjp (control-derivative golden, central differences) and the assembled sensitivity goldens."""
     conn = np.asarray(conn, dtype=np.int64)
     X = np.asarray(coords, dtype=np.float64)[conn]
     f = np.asarray(field, dtype=np.float64)
@@ -666,7 +668,10 @@ def global_residual_shape_vjp(ph, coords, conn, field, u, w, h=None):
 
     Thermal and linear-elastic elements are differentiated by complex step (exact to rounding: the
     geometry keeps complex coordinates).  The Neo-Hookean oracle evaluates psi/S in long double, which
-    has no complex form, so it takes central differences (step h, accuracy ~1e-9 relative)."""
+    has no complex form, so it takes central differences (step h, accuracy ~1e-9 relative).
+    Pinned by the reference's shape-derivative goldens (test_sensitivity_analysis.py,
+    test_mechanical_2D_sa.py), by central differences of the assembled residual and by translation
+    invariance (the columns of a rigid shift sum to zero)."""
     coords = np.asarray(coords, dtype=np.float64)
     conn = np.asarray(conn)
     field = np.asarray(field, dtype=np.float64)
