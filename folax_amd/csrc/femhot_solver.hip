@@ -98,9 +98,11 @@ __global__ __launch_bounds__(256) void k_spmv_node(long long nnodes, const long 
 #pragma unroll
     for (int c = 0; c < P; ++c) rb[c] = rp[node * P + c];
     const int deg = (int)((rp[node * P + 1] - rb[0]) / P);
-    double s[P];
+    // accumulated in the arithmetic type: FP32 for the inner iterations of the mixed solve (whose
+    // FP64 outer refinement recomputes the residual), FP64 otherwise
+    T s[P];
 #pragma unroll
-    for (int c = 0; c < P; ++c) s[c] = 0.0;
+    for (int c = 0; c < P; ++c) s[c] = T(0);
     for (int k0 = lane; k0 < deg; k0 += U * LPN) {
       int col[U];
       T vv[U][P][P];
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(256) void k_spmv_node(long long nnodes, const long 
 #pragma unroll
         for (int c = 0; c < P; ++c)
 #pragma unroll
-          for (int e = 0; e < P; ++e) s[c] += (double)vv[u][c][e] * (double)xv[e];
+          for (int e = 0; e < P; ++e) s[c] += vv[u][c][e] * xv[e];
       }
     }
 #pragma unroll
@@ -132,8 +134,8 @@ __global__ __launch_bounds__(256) void k_spmv_node(long long nnodes, const long 
     if (lane == 0) {
 #pragma unroll
       for (int c = 0; c < P; ++c) {
-        y[node * P + c] = (T)s[c];
-        d += (double)x[node * P + c] * (double)(T)s[c];
+        y[node * P + c] = s[c];
+        d += (double)x[node * P + c] * (double)s[c];
       }
     }
   }
