@@ -62,15 +62,16 @@ __global__ __launch_bounds__(256) void k_spmv(long long nrows, const long long *
       cols[u] = in ? ci[k] : (int)row;
       vals[u] = in ? v[k] : T(0);
     }
-    double s = 0.0;
+    // accumulated in the arithmetic type (FP32 only in the mixed solve's inner iterations)
+    T s = T(0);
 #pragma unroll
-    for (int u = 0; u < U; ++u) s += (double)vals[u] * (double)x[cols[u]];
-    for (long long k = b + lane + (long long)U * LPR; k < e; k += LPR) s += (double)v[k] * (double)x[ci[k]];
+    for (int u = 0; u < U; ++u) s += vals[u] * x[cols[u]];
+    for (long long k = b + lane + (long long)U * LPR; k < e; k += LPR) s += v[k] * x[ci[k]];
 #pragma unroll
     for (int off = LPR / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off, LPR);
     if (lane == 0) {
-      y[row] = (T)s;
-      d = (double)x[row] * (double)(T)s;
+      y[row] = s;
+      d = (double)x[row] * (double)s;
     }
   }
   if (part) {
