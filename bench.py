@@ -87,7 +87,8 @@ def cpu_baseline(W, budget_s=10.0):
 def solver_profile(iters=200):
     """HEX8 64^3 linear elasticity (274,625 nodes, 823,875 dofs), E in {1, 10} from a seeded Gaussian random
     field (correlation length 8 voxels), clamped at z = 0 with 1 % stretch at z = 1: FP64 tangent + residual
-    (colour-ordered assembly), then `iters` FP64 Jacobi-PCG iterations with the node-blocked SpMV."""
+    (colour-ordered assembly), then `iters` FP64 Jacobi-PCG iterations with the node-blocked SpMV, then
+    `iters` iterations of the mixed solve (FP32 inner iterations, FP64 refinement) on the same system."""
     import torch
     from folax_amd import loss_functions as lf
     from folax_amd.controls import grf_two_phase
@@ -105,6 +106,10 @@ def solver_profile(iters=200):
                                  raise_on_fail=False, context_pattern=True)
     torch.cuda.synchronize()
     log(f"solver profile: HEX8 64^3 tangent nnz {K.values.numel()}, {it} FP64 PCG iterations, rel residual {rr:.3e}")
+    x, it, rr, ok = loss.ctx.pcg(K.row_ptr, K.col_idx, K.values, (-R).contiguous(), tol=1e-14, maxiter=iters,
+                                 mode="mixed", raise_on_fail=False, context_pattern=True)
+    torch.cuda.synchronize()
+    log(f"solver profile: {it} mixed PCG iterations, rel residual {rr:.3e}")
 
 
 def main():
@@ -120,7 +125,7 @@ def main():
     ap.add_argument("--grid", type=int, default=0, help="workgroups per CU (0: one per chunk)")
     ap.add_argument("--solver-profile", action="store_true",
                     help="short run for rocprofv3 of the linear-solver kernels instead: HEX8 64^3 two-phase "
-                         "elastic tangent (colour-ordered FP64 assembly) + 200 FP64 PCG iterations")
+                         "elastic tangent (colour-ordered FP64 assembly) + 200 FP64 and 200 mixed PCG iterations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the library has no multi-GPU path)",
