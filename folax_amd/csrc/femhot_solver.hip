@@ -104,28 +104,31 @@ __global__ __launch_bounds__(256) void k_spmv_node(long long nnodes, const long 
     T s[P];
 #pragma unroll
     for (int c = 0; c < P; ++c) s[c] = T(0);
+    // slots past the node's last neighbour re-read that neighbour (deg >= 1 inside the loop) and skip
+    // the add: unconditional loads let each row's P consecutive values merge into one wide load
     for (int k0 = lane; k0 < deg; k0 += U * LPN) {
       int col[U];
       T vv[U][P][P];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int k = k0 + u * LPN;
-        const bool in = k < deg;
-        col[u] = in ? ci[rb[0] + (long long)P * k] : (int)(node * P);
+        const long long k = min(k0 + u * LPN, deg - 1);
+        col[u] = ci[rb[0] + P * k];
 #pragma unroll
         for (int c = 0; c < P; ++c)
 #pragma unroll
-          for (int e = 0; e < P; ++e) vv[u][c][e] = in ? v[rb[c] + (long long)P * k + e] : T(0);
+          for (int e = 0; e < P; ++e) vv[u][c][e] = v[rb[c] + P * k + e];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         T xv[P];
 #pragma unroll
         for (int e = 0; e < P; ++e) xv[e] = x[col[u] + e];
+        if (k0 + u * LPN < deg) {
 #pragma unroll
-        for (int c = 0; c < P; ++c)
+          for (int c = 0; c < P; ++c)
 #pragma unroll
-          for (int e = 0; e < P; ++e) s[c] += vv[u][c][e] * xv[e];
+            for (int e = 0; e < P; ++e) s[c] += vv[u][c][e] * xv[e];
+        }
       }
     }
 #pragma unroll
