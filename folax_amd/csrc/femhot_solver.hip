@@ -214,33 +214,42 @@ __global__ __launch_bounds__(256) void k_init(long long n, const double *b, T *r
   if (threadIdx.x == 0) rz_part[blockIdx.x] = rz, rr_part[blockIdx.x] = rr;
 }
 
-// alpha = (r.z) / (p.Ap); x += alpha p; r -= alpha q; partials of the new r.z and r.r
+// alpha = (r.z) / (p.Ap); x += alpha p; r -= alpha q; partials of the new r.z and r.r.  The
+// thread's first element is loaded before the partial sums, so its loads overlap their latency.
 template <class T>
 __global__ __launch_bounds__(256) void k_update_xr(long long n, T *x, T *r, const T *p, const T *q, const T *dinv,
                                                    const double *rz_part, const double *pap, double *rz_next,
                                                    double *rr_next) {
+  const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
+  const long long j0 = i0 < n ? i0 : 0;  // n >= 1: index 0 is in bounds
+  const T x0 = x[j0], p0 = p[j0], r0 = r[j0], q0 = q[j0], d0 = dinv[j0];
   const double alpha = sum_partials(rz_part) / pap[0];
   double rz = 0.0, rr = 0.0;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-    x[i] = (T)((double)x[i] + alpha * (double)p[i]);
-    const T ri = (T)((double)r[i] - alpha * (double)q[i]);
+  auto step = [&](long long i, T xi, T pi, T rv, T qi, T di) {
+    x[i] = (T)((double)xi + alpha * (double)pi);
+    const T ri = (T)((double)rv - alpha * (double)qi);
     r[i] = ri;
-    const T z = ri * dinv[i];
+    const T z = ri * di;
     rz += (double)ri * (double)z;
     rr += (double)ri * (double)ri;
-  }
+  };
+  if (i0 < n) step(i0, x0, p0, r0, q0, d0);
+  for (long long i = i0 + st; i < n; i += st) step(i, x[i], p[i], r[i], q[i], dinv[i]);
   rz = block_sum256(rz);
   rr = block_sum256(rr);
   if (threadIdx.x == 0) rz_next[blockIdx.x] = rz, rr_next[blockIdx.x] = rr;
 }
 
-// beta = (r.z)_new / (r.z)_old; p = Dinv r + beta p
+// beta = (r.z)_new / (r.z)_old; p = Dinv r + beta p (first element loaded before the partial sums)
 template <class T>
 __global__ __launch_bounds__(256) void k_update_p(long long n, T *p, const T *r, const T *dinv, const double *rz_new,
                                                   const double *rz_old) {
+  const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
+  const long long j0 = i0 < n ? i0 : 0;  // n >= 1: index 0 is in bounds
+  const T r0 = r[j0], d0 = dinv[j0], p0 = p[j0];
   const double beta = sum_partials(rz_new) / sum_partials(rz_old);
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    p[i] = (T)((double)(r[i] * dinv[i]) + beta * (double)p[i]);
+  if (i0 < n) p[i0] = (T)((double)(r0 * d0) + beta * (double)p0);
+  for (long long i = i0 + st; i < n; i += st) p[i] = (T)((double)(r[i] * dinv[i]) + beta * (double)p[i]);
 }
 
 template <class T> __global__ void k_axpy_to64(long long n, const T *d, double *x) {
