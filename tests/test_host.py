@@ -106,3 +106,13 @@ def test_loss_dirichlet_bookkeeping_matches_oracle():
     assert np.array_equal(loss.non_dirichlet_indices, fe.free_dofs(32, idx))
     with pytest.raises(ValueError):
         lf.ThermalLoss2DQuad("t", {}, m)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says why, before touching a GPU."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
