@@ -8,7 +8,7 @@
 //   geometry      fol/geometries/geometry.py:87-97, quadrilateral_2d_4.py:17-66,
 //                 hexahedra_3d_8.py:17-114, tetrahedra_3d_4.py:17-60
 //   thermal       fol/loss_functions/thermal.py:28-49
-//   linear elast. fol/loss_functions/mechanical.py:115-195
+//   linear elast. fol/loss_functions/mechanical.py:14-117
 //   Neo-Hookean   fol/loss_functions/mechanical_neohooke.py:49-91,171-275 and
 //                 fol/constitutive_material_models/neo_hooke.py:15-109
 #pragma once
@@ -280,8 +280,8 @@ struct PhysParams {
   double body[3];
   int has_body;  // any body[k] != 0 (the body-force terms are skipped otherwise)
   // derived on the host
-  double c2, c3, c4;      // 3-D D matrix entries for young_modulus E (mechanical.py:141-160)
-  double f2, f2nu, f2sh;  // 2-D plane stress entries (mechanical.py:138-139)
+  double c2, c3, c4;      // 3-D D matrix entries for young_modulus E (mechanical.py:63-82)
+  double f2, f2nu, f2sh;  // 2-D plane stress entries (mechanical.py:60-61)
   double kap1, mu1;       // Neo-Hookean moduli per unit field value (mechanical_neohooke.py:253-255)
 };
 

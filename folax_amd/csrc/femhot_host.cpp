@@ -538,7 +538,7 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
   } else {
     c.h_seg_off = {0, c.Ne};
   }
-  // physics parameters (mechanical.py:138-160, mechanical_neohooke.py:253-255)
+  // physics parameters (mechanical.py:60-82, mechanical_neohooke.py:253-255)
   PhysParams &p = c.pp;
   p.E = d->young_modulus;
   p.nu = d->poisson_ratio;

@@ -1,6 +1,6 @@
 // Element routines of the three physics on register-resident element data.
 //   residual      r_e and the reference element energy (thermal T.r, mechanical u.r,
-//                 Neo-Hookean Ee)                           thermal.py:28-49, mechanical.py:176-195,
+//                 Neo-Hookean Ee)                           thermal.py:28-49, mechanical.py:98-117,
 //                                                           mechanical_neohooke.py:243-275
 //   vjp           g_u = (dr_e/du)^T w, g_d = (dr_e/dd)^T w  (fe_response.py:434-453 contracts the
 //                 same jacobian), recomputing Gauss-point quantities
@@ -47,8 +47,8 @@ __device__ __forceinline__ void nodal_grad(const R (&v)[N * D], const R (&DN)[N]
 }
 
 // Linear-elastic stress tensor (unit field value) from a displacement gradient.
-// 3-D Voigt [xx,yy,zz,xy,yz,xz] with engineering shears (mechanical.py:46-58, 141-160);
-// 2-D plane stress [xx,yy,xy] (mechanical.py:37-44, 138-139).
+// 3-D Voigt [xx,yy,zz,xy,yz,xz] with engineering shears (mechanical.py:46-58, 63-82);
+// 2-D plane stress [xx,yy,xy] (mechanical.py:37-44, 60-61).
 template <class R, int D>
 __device__ __forceinline__ void lin_stress(const R (&G)[D][D], const PhysParams &pp, R (&S)[D][D]) {
   if constexpr (D == 3) {

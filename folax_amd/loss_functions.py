@@ -260,10 +260,14 @@ class _FullDofs(torch.autograd.Function):
 
 
 class _BatchEnergy(torch.autograd.Function):
-    """Per-sample energies of ComputeBatchLoss.  Backward follows jax.grad of the reference loss:
-    thermal/mechanical losses use u^T stop_gradient(r) (thermal.py:45-49, mechanical.py:194-195), so
-    dE/du = r and dE/dcontrol = 0; Neo-Hookean Ee has no stop_gradient, so dE/du = Fint and
-    dE/dcontrol = dEe/dcontrol (energy-gradient kernel)."""
+    """Per-sample energies of ComputeBatchLoss.  Backward follows jax.grad of the reference loss.
+    * Linear elastic: E = u^T stop_gradient(Se u - Fe) (mechanical.py:116-117), so dE/du = r and
+      dE/dcontrol = 0.
+    * Thermal: E = T^T (Se(k0, sg(T)) sg(T) - Fe) (thermal.py:31, :44-49).  Only T is under
+      stop_gradient, not Se, so dE/dT = r and dE/dk0 = T^T (dSe/dk0) T, which is the residual VJP's
+      field part with the cotangent w = T.
+    * Neo-Hookean: Ee has no stop_gradient, so dE/du = Fint and dE/dcontrol = dEe/dcontrol
+      (energy-gradient kernel)."""
 
     @staticmethod
     def forward(ctx, loss, params, dofs):
@@ -289,7 +293,10 @@ class _BatchEnergy(torch.autograd.Function):
                 gu = gu - r0
         else:
             gu, _ = loss.ctx.residual(params, dofs, want_energy=False, flags=flags)
-            gd = torch.zeros_like(params)
+            if loss.physics == L.THERMAL:
+                _, gd = loss.ctx.residual_vjp(params, dofs, dofs, want_gu=False, flags=flags)
+            else:
+                gd = torch.zeros_like(params)
         s = gE.to(dofs.dtype).reshape(-1, 1)
         return None, gd * s, gu * s
 
@@ -351,7 +358,7 @@ class ThermalLoss3DTetra(ThermalLoss):
 
 
 class MechanicalLoss(FiniteElementLoss):
-    """mechanical.py:11-195: small-strain isotropic elasticity, D from material_dict scaled by the
+    """mechanical.py:14-117: small-strain isotropic elasticity, D from material_dict scaled by the
     nodal control field; 2-D is plane stress.  body force key `body_foce` as in the reference."""
 
     physics = L.MECHANICAL

@@ -5,7 +5,7 @@
  * load this library; the product package never does.
  *
  * tet4_elastic_energy_grad: small-strain isotropic TET4 element (one-point rule, w = 1/6,
- * fol/geometries/tetrahedra_3d_4.py:17-60; D of fol/loss_functions/mechanical.py:141-160 scaled by a
+ * fol/geometries/tetrahedra_3d_4.py:17-60; D of fol/loss_functions/mechanical.py:63-82 scaled by a
  * per-element field; B rows [xx,yy,zz,xy,yz,xz] of mechanical.py:46-58), the element potential
  * Pi_e = 1/2 u_e^T K_e u_e, its gradient r_e = K_e u_e scattered into R with OpenMP atomics (the
  * scatter pattern of fol/loss_functions/ffi_functions/kr_small_displacement_element.cc:81-93) and

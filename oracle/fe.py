@@ -121,7 +121,7 @@ def geometry(etype: str, X, num_gp: int):
     """Per element and Gauss point: N (g,n), weights (g,), global gradients DN (e,g,n,dim), signed detJ (e,g).
 
     DN = dN/dxi . inv(J) (geometry.py:87-92); detJ = jnp.linalg.det(J), no abs
-    (thermal.py:37, mechanical.py:185).
+    (thermal.py:37, mechanical.py:107).
     """
     pts, wts = gauss_rule(etype, num_gp)
     dim = ELEM_DIM[etype]
@@ -153,7 +153,7 @@ def _field_weights(N, d_ndim, Ne):
 # --------------------------------------------------------------------------
 
 def _body_load(N, w, detJ, body, ndpn):
-    """Fe[a*ndpn+c] = sum_g w detJ N_a b_c (thermal.py:39 with scalar b; mechanical.py:187 N_mat^T b)."""
+    """Fe[a*ndpn+c] = sum_g w detJ N_a b_c (thermal.py:39 with scalar b; mechanical.py:109 N_mat^T b)."""
     body = np.atleast_1d(np.asarray(body, dtype=np.float64)).reshape(-1)
     if body.size == 1 and ndpn > 1:
         body = np.repeat(body, ndpn)
@@ -162,18 +162,27 @@ def _body_load(N, w, detJ, body, ndpn):
     return (fe[:, :, None] * body[None, None, :ndpn]).reshape(Ne, -1)
 
 
+def conductivity_at_gp(N, d, T, beta, c):
+    """k at each Gauss point (thermal.py:31-33): the field and the temperature are interpolated
+    first, k = (N.d) (1 + beta (N.T)^c).  Returns (k, N.d, N.T), each (e, g).
+    Pinned by the Kirchhoff closed forms (tests/test_oracle_pins.py): exact element residuals for
+    fields linear along x, and the discrete strip solution Phi(T_i) linear in x_i."""
+    dg = _field_at_gp(N, d)
+    Tg = np.einsum("ga,ea->eg", N, T)
+    return dg * (1.0 + beta * Tg ** c), dg, Tg
+
+
 def thermal_element(etype, X, d, T, num_gp, beta=0.0, c=1.0, body=0.0, want_K=True):
     """ThermalLoss.ComputeElement (fol/loss_functions/thermal.py:28-49).
 
     k_gp = (N.d)(1 + beta (N.T)^c) with T under stop_gradient, Se = sum k DN DN^T detJ w,
     Fe = sum w detJ b N.  Returns (energy = T.(Se T - Fe), r = Se T - Fe, Se).
-    Pinned by: unit-square closed form, patch test, R = d/dT(1/2 T^T Se T) at beta=0.
+    Pinned by: unit-square closed form, patch test, R = d/dT(1/2 T^T Se T) at beta=0, and the
+    Kirchhoff closed forms of beta != 0 (exact element residuals, exact strip solution).
     """
     N, w, DN, detJ = geometry(etype, X, num_gp)
     T = np.asarray(T, dtype=np.float64)
-    dg = _field_at_gp(N, d)
-    Tg = np.einsum("ga,ea->eg", N, T)
-    k = dg * (1.0 + beta * Tg ** c)
+    k, _, _ = conductivity_at_gp(N, d, T, beta, c)
     coef = k * w[None, :] * detJ
     Se = np.einsum("eg,egai,egbi->eab", coef, DN, DN)
     Fe = _body_load(N, w, detJ, body, 1)
@@ -183,7 +192,7 @@ def thermal_element(etype, X, d, T, num_gp, beta=0.0, c=1.0, body=0.0, want_K=Tr
 
 
 def d_matrix(dim, E, nu):
-    """Isotropic D: 2-D plane stress (mechanical.py:138-139), 3-D (mechanical.py:141-160)."""
+    """Isotropic D: 2-D plane stress (mechanical.py:60-61), 3-D (mechanical.py:63-82)."""
     if dim == 2:
         return np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]], dtype=np.float64) * (E / (1 - nu ** 2))
     c1 = E / ((1.0 + nu) * (1.0 - 2.0 * nu))
@@ -221,7 +230,7 @@ def b_matrix(DN):
 
 
 def mech_element(etype, X, d, u, num_gp, E=1.0, nu=0.3, body=None, want_K=True):
-    """MechanicalLoss.ComputeElement (fol/loss_functions/mechanical.py:176-195).
+    """MechanicalLoss.ComputeElement (fol/loss_functions/mechanical.py:98-117).
 
     Se = sum w detJ (N.d) B^T D B, Fe = sum w detJ N_mat^T b, returns
     (u.(Se u - Fe), Se u - Fe, Se).  Pinned by tests/unit/test_mechanical_loss.py goldens,
@@ -242,6 +251,11 @@ def mech_element(etype, X, d, u, num_gp, E=1.0, nu=0.3, body=None, want_K=True):
 
 
 _VOIGT = {3: [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)], 2: [(0, 0), (1, 1), (0, 1)]}
+
+# The isochoric part of the Neo-Hookean energy scales trC by J^(ISO_J_POWER / dim)
+# (neo_hooke.py:77-80 for 3-D, :29-32 for 2-D).  A module constant so the pins of
+# tests/test_oracle_pins.py can plant a wrong exponent and see it caught.
+ISO_J_POWER = -2.0
 
 
 def nh_material(F, kappa, mu, want_C=True):
@@ -268,7 +282,7 @@ def nh_material(F, kappa, mu, want_C=True):
     JL = _det_ld(FL)
     invCL = _adj_ld(CL) / (JL * JL)[..., None, None]
     trCL = np.trace(CL, axis1=-2, axis2=-1)
-    aL = JL ** (LD(-2) / LD(dim))
+    aL = JL ** (LD(ISO_J_POWER) / LD(dim))
     psiL = (kL / 4) * (JL ** 2 - 2 * np.log(JL) - 1) + mL / 2 * (aL * trCL - dim)
     pL = kL / 2 * (JL - 1 / JL)
     SisoL = (aL * mL)[..., None, None] * (np.eye(dim, dtype=LD) - (trCL / dim)[..., None, None] * invCL)
@@ -444,9 +458,7 @@ def residual_vjp(ph: Physics, X, d, u, wv, picard=False):
     Nd = _field_weights(N, d.ndim, X.shape[0])
     if ph.kind == "thermal":
         T = np.asarray(u, dtype=np.float64)
-        dg = _field_at_gp(N, d)
-        Tg = np.einsum("ga,ea->eg", N, T)
-        k = dg * (1.0 + ph.beta * Tg ** ph.c)
+        k, dg, Tg = conductivity_at_gp(N, d, T, ph.beta, ph.c)
         gw = np.einsum("egai,ea->egi", DN, wv)
         gT = np.einsum("egai,ea->egi", DN, T)
         q = wd * np.einsum("egi,egi->eg", gw, gT)  # w^T G_g T
@@ -484,7 +496,7 @@ def energy_grad(ph: Physics, X, d, u):
     """Element potential Pi_e and its gradient (g_u = dPi/du = r_e, g_d = dPi/dd).
 
     mechanical: Pi = 1/2 u^T Se u - u^T Fe (the reference's loss u^T stop_grad(Se u - Fe),
-      mechanical.py:194-195, has jax.grad_u = r_e as here, and jax.grad_d = 0 by stop_gradient);
+      mechanical.py:116-117, has jax.grad_u = r_e as here, and jax.grad_d = 0 by stop_gradient);
     thermal (beta = 0 only): Pi = 1/2 T^T Se T - T^T Fe;
     neohooke: Pi = Ee - u^T Fe with Ee of mechanical_neohooke.py:266-275 (no stop_gradient:
       jax.grad of ComputeTotalEnergy equals this gradient when the body force is zero).

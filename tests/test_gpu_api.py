@@ -142,7 +142,8 @@ def test_c1_thermal_quad4_40x40_fp64():
 @pytest.mark.parametrize("exponent", [1.0, 2.0])
 def test_batch_loss_value_and_autograd(kind, exponent):
     """ComputeBatchLoss: value = mean_b (sum_e E_e)^p; gradients follow jax.grad of the reference
-    (stop_gradient semantics for thermal / linear elastic, full derivative for Neo-Hookean)."""
+    (dE/du = r for thermal and linear elastic; dE/dk0 = T^T dSe/dk0 T for thermal, zero for linear
+    elastic; the full derivative for Neo-Hookean)."""
     from folax_amd import loss_functions as lf
     from folax_amd.mesh import create_3D_box_mesh
     m = create_3D_box_mesh(3, 3, 3)
@@ -181,13 +182,24 @@ def test_batch_loss_value_and_autograd(kind, exponent):
         else:
             _, gu_ref = fe.assemble(ph, coords, conn, Kf[b], Ub[b], np.zeros(0, np.int64), want_K=False)
             gd_ref = np.zeros(coords.shape[0])
+        if kind == "thermal":
+            # thermal.py:31 freezes T only: jax.grad w.r.t. k0 is the plain derivative of the
+            # reference energy T^T (Se(k0, T) T - Fe), linear in k0, so central differences are exact
+            h = 1e-3
+            for a in range(coords.shape[0]):
+                kp, km = Kf[b].copy(), Kf[b].copy()
+                kp[a] += h
+                km[a] -= h
+                gd_ref[a] = (fe.total_energy(ph, coords, conn, kp, Ub[b]) -
+                             fe.total_energy(ph, coords, conn, km, Ub[b])) / (2 * h)
+            assert np.abs(gd_ref).max() > 1e-3
         gu_ref = gu_ref.copy()
         gu_ref[idx] = 0.0
         assert rel(Ut.grad[b].cpu().numpy(), scale * gu_ref) <= 1e-11
-        if kind == "neohooke":
-            assert rel(Pt.grad[b].cpu().numpy(), scale * gd_ref) <= 1e-11
-        else:
+        if kind == "mechanical":  # mechanical.py:116: the whole residual is under stop_gradient
             assert float(Pt.grad[b].abs().max()) == 0.0
+        else:
+            assert rel(Pt.grad[b].cpu().numpy(), scale * gd_ref) <= (1e-9 if kind == "thermal" else 1e-11)
 
 
 @pytest.mark.parametrize("autotune", [False, True])
