@@ -36,9 +36,18 @@ EXPORTED = [
     "femhot_pattern", "femhot_colouring", "femhot_residual", "femhot_tangent_csr", "femhot_residual_vjp",
     "femhot_energy_grad", "femhot_pcg", "femhot_spmv", "femhot_orient_elements", "femhot_autotune",
     "femhot_kernel_timing", "femhot_set_chunk_config", "femhot_residual_shape_vjp", "femhot_newton",
+    "femhot_gauss_point_data",
 ]
 
-OP_RESIDUAL, OP_VJP, OP_ENERGY_GRAD = 0, 1, 2
+OP_RESIDUAL, OP_VJP, OP_ENERGY_GRAD, OP_TANGENT = 0, 1, 2, 3
+PATH_ATOMIC, PATH_COLOUR = 1, 2  # autotune's block when an element-parallel path won (epc = 0)
+
+
+def gauss_points_per_element(elem_type, num_gp):
+    """Points of Gauss rule num_gp (1/2/3) per element (femhot_gauss_point_data's G)."""
+    if elem_type == TET4:
+        return {1: 1, 2: 4, 3: 8}[num_gp]
+    return num_gp ** (2 if elem_type == QUAD4 else 3)
 
 
 class FemhotError(RuntimeError):
@@ -93,6 +102,7 @@ def lib():
     L.femhot_residual_vjp.argtypes = [vp, i32, i32, vp, ll, vp, ll, vp, ll, vp, ll, vp, ll, u32, vp]
     L.femhot_energy_grad.argtypes = [vp, i32, i32, vp, ll, vp, ll, vp, vp, ll, vp, ll, u32, vp]
     L.femhot_residual_shape_vjp.argtypes = [vp, i32, vp, vp, vp, vp, u32, vp]
+    L.femhot_gauss_point_data.argtypes = [vp, i32, vp, vp, vp, vp]
     L.femhot_pcg.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, i32, C.POINTER(i32), C.POINTER(dbl), vp]
     L.femhot_spmv.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.femhot_newton.argtypes = [vp, i32, vp, vp, u32, i32, i32, dbl, dbl, dbl, i32, C.POINTER(dbl), C.POINTER(i32), vp]

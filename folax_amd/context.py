@@ -195,6 +195,20 @@ class FemhotContext:
                                                    flags, _stream()))
         return gx
 
+    def gauss_point_data(self, num_gp, dtype=torch.float64, device=None):
+        """Integration data at the context's Gauss rule: N (G, n), w detJ (n_elems, G) and the global
+        shape gradients dN/dx (n_elems, G, n, dim), from the library's element geometry."""
+        device = device or torch.device("cuda")
+        G = L.gauss_points_per_element(self.et, num_gp)
+        dim = 2 if self.et == L.QUAD4 else 3
+        n = self.nodes_per_elem
+        N = torch.empty(G, n, dtype=dtype, device=device)
+        wd = torch.empty(self.n_elems, G, dtype=dtype, device=device)
+        dn = torch.empty(self.n_elems, G, n, dim, dtype=dtype, device=device)
+        code = L.F32 if dtype == torch.float32 else L.F64
+        L.check(self._lib.femhot_gauss_point_data(self._h, code, _ptr(N), _ptr(wd), _ptr(dn), _stream()))
+        return N, wd, dn
+
     def set_chunk_config(self, op, dtype, batch, epc, block, grid_per_cu=0, flags=0):
         code = L.F32 if dtype == torch.float32 else L.F64
         L.check(self._lib.femhot_set_chunk_config(self._h, int(op), code, int(batch), int(flags), int(epc),
@@ -231,6 +245,18 @@ class FemhotContext:
                                           ws, _ptr(ou), self.n_dofs, _ptr(od), fsz, _ptr(en), flags, C.byref(epc),
                                           C.byref(blk), C.byref(grid), C.byref(ms), _stream()))
         return epc.value, blk.value, grid.value, ms.value
+
+    def autotune_tangent(self, field, u, flags=0):
+        """Times atomic against colour-ordered tangent assembly on these inputs and keeps the faster
+        for tangent_csr calls with SCATTER_CHUNK.  Returns (L.PATH_ATOMIC or L.PATH_COLOUR, ms)."""
+        field = field.reshape(-1).contiguous()
+        u = u.reshape(-1).contiguous()
+        vals = torch.empty(max(self.nnz(), 1), dtype=u.dtype, device=u.device)
+        epc, blk, grid, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        L.check(self._lib.femhot_autotune(self._h, L.OP_TANGENT, _dtype_code(u), 1, _ptr(field), 0, _ptr(u), 0, None,
+                                          0, _ptr(vals), 0, None, 0, None, int(flags), C.byref(epc), C.byref(blk),
+                                          C.byref(grid), C.byref(ms), _stream()))
+        return blk.value, ms.value
 
     def tangent_csr(self, field, u, want_R=True, flags=0):
         """CSR values (nnz) on the pattern of `pattern()`, BC-masked, and the masked residual."""

@@ -38,6 +38,9 @@ class FourierControl:
         self.num_controlled_vars = self.fe_mesh.GetNumberOfNodes()
         self.initialized = True
 
+    def GetName(self):
+        return self.name
+
     def GetNumberOfVariables(self):
         return self.num_control_vars
 

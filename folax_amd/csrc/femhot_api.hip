@@ -57,13 +57,19 @@ static int validate(Ctx *c, const CallIO &io, const char *fn) {
   return FEMHOT_OK;
 }
 
-static int run_elem(Ctx *c, const CallIO &io) {
+static int run_elem(Ctx *c, const CallIO &io_in) {
+  CallIO io = io_in;
   const bool elemental = (io.flags & FEMHOT_ELEMENT_FIELD) != 0;
   const long long fsize = elemental ? c->Ne : c->Nn;
   const bool acc = (io.flags & FEMHOT_ACCUMULATE) != 0;
   int rc;
-  const bool chunk = (io.flags & FEMHOT_SCATTER_CHUNK) != 0 &&
-                     !chunk_prefers_plain(c, io.dtype, io.mode, io.batch, io.flags);
+  // FEMHOT_SCATTER_CHUNK leaves the scatter to the library: the path tuned for this call shape
+  bool chunk = false;
+  if (io.flags & FEMHOT_SCATTER_CHUNK) {
+    const int path = tuned_path(c, io.dtype, io.mode, io.batch, io.flags);
+    chunk = path == PATH_CHUNK;
+    if (path == PATH_COLOUR) io.flags |= FEMHOT_SCATTER_COLOUR;
+  }
   // clear outputs (the chunk path writes every node itself and clears only shared nodes)
   if (!acc && !chunk) {
     if (io.dtype == FEMHOT_F32) {
@@ -225,6 +231,11 @@ int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const bool acc = (flags & FEMHOT_ACCUMULATE) != 0;
+  // FEMHOT_SCATTER_CHUNK leaves the assembly order to the library: the path femhot_autotune
+  // measured fastest for the tangent (colour-ordered untuned)
+  if ((flags & FEMHOT_SCATTER_CHUNK) && !(flags & FEMHOT_SCATTER_COLOUR) &&
+      tuned_path(c, dtype, MODE_TANGENT, 1, flags) == PATH_COLOUR)
+    flags |= FEMHOT_SCATTER_COLOUR;
   if (!acc) {
     rc = dtype == FEMHOT_F32 ? fill<float>((float *)vals, c->nnz, c->nnz, 1, 0.f, s)
                              : fill<double>((double *)vals, c->nnz, c->nnz, 1, 0.0, s);
@@ -247,7 +258,9 @@ int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *
   A.flags = flags;
   A.pp = c->pp;
   const int n = c->n;
-  if (flags & FEMHOT_SCATTER_COLOUR) {
+  // colour-ordered plain adds need distinct nodes within an element (lanes of one element would
+  // otherwise update the same entries); meshes with collapsed elements assemble with atomics
+  if ((flags & FEMHOT_SCATTER_COLOUR) && !c->repeated_nodes) {
     if ((rc = ensure_colouring(c))) return rc;
     L.plain = true;
     for (int k = 0; k < c->n_colours; ++k) {
@@ -285,9 +298,46 @@ int femhot_autotune(femhot_ctx *h, int op, int dtype, int batch, const void *fie
                     long long ofs, double *energy, unsigned flags, int *epc, int *block, int *grid_per_cu,
                     float *ms, void *stream) {
   FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
-  FH_CHECK_ARG(op >= FEMHOT_OP_RESIDUAL && op <= FEMHOT_OP_ENERGY_GRAD, FEMHOT_ERR_ARG, "femhot_autotune: bad op");
+  FH_CHECK_ARG(op >= FEMHOT_OP_RESIDUAL && op <= FEMHOT_OP_TANGENT, FEMHOT_ERR_ARG, "femhot_autotune: bad op");
   FH_CHECK_ARG(op != FEMHOT_OP_VJP || w, FEMHOT_ERR_ARG, "femhot_autotune: VJP needs a cotangent");
   Ctx *c = &h->c;
+  hipEvent_t e0, e1;
+  // mean time of `reps` calls of f after one warm-up call, with HIP events on the stream
+  auto time_it = [&](auto &&f, float *ms_out) -> int {
+    int r = f();
+    if (r) return r;
+    FH_HIP(hipEventRecord(e0, (hipStream_t)stream));
+    constexpr int reps = 3;
+    for (int k = 0; k < reps && !r; ++k) r = f();
+    FH_HIP(hipEventRecord(e1, (hipStream_t)stream));
+    FH_HIP(hipEventSynchronize(e1));
+    if (r) return r;
+    float t = 0.f;
+    FH_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms_out = t / reps;
+    return FEMHOT_OK;
+  };
+  if (op == FEMHOT_OP_TANGENT) {  // atomic against colour-ordered assembly; out_u holds the values
+    FH_CHECK_ARG(field && u && out_u, FEMHOT_ERR_ARG, "femhot_autotune: tangent needs field, u and values");
+    const unsigned tf = flags & (FEMHOT_ELEMENT_FIELD | FEMHOT_TRANSPOSE);
+    FH_HIP(hipEventCreate(&e0));
+    FH_HIP(hipEventCreate(&e1));
+    float t_atomic = 0.f, t_colour = 0.f;
+    int rc = time_it([&] { return femhot_tangent_csr(h, dtype, field, u, out_u, nullptr, tf, stream); }, &t_atomic);
+    if (!rc)
+      rc = time_it([&] { return femhot_tangent_csr(h, dtype, field, u, out_u, nullptr, tf | FEMHOT_SCATTER_COLOUR,
+                                                   stream); }, &t_colour);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    const int path = t_colour <= t_atomic ? PATH_COLOUR : PATH_ATOMIC;
+    record_path(c, dtype, MODE_TANGENT, 1, flags, path);
+    if (epc) *epc = 0;
+    if (block) *block = path == PATH_COLOUR ? FEMHOT_PATH_COLOUR : FEMHOT_PATH_ATOMIC;
+    if (grid_per_cu) *grid_per_cu = 0;
+    if (ms) *ms = std::min(t_colour, t_atomic);
+    return FEMHOT_OK;
+  }
   CallIO io;
   io.dtype = dtype;
   io.batch = batch;
@@ -312,39 +362,38 @@ int femhot_autotune(femhot_ctx *h, int op, int dtype, int batch, const void *fie
                       io.batch > 1 ? ws : 0, out_u, ous, out_field, ofs, energy, io.flags, io.stream, epc, block,
                       grid_per_cu, &chunk_ms);
   if (rc) return rc;
-  // the plain path competes too; when it wins, FEMHOT_SCATTER_CHUNK calls of this shape take it
+  // the element-parallel paths compete too (hardware atomics, and colour-ordered plain adds); when
+  // one wins, FEMHOT_SCATTER_CHUNK calls of this shape take it
   CallIO pio = io;
   pio.flags &= ~FEMHOT_SCATTER_CHUNK;
-  hipEvent_t e0, e1;
+  CallIO cio = pio;
+  cio.flags |= FEMHOT_SCATTER_COLOUR;
   FH_HIP(hipEventCreate(&e0));
   FH_HIP(hipEventCreate(&e1));
-  if ((rc = run_elem(c, pio))) return rc;
-  FH_HIP(hipEventRecord(e0, io.stream));
-  constexpr int reps = 3;
-  for (int r = 0; r < reps && !rc; ++r) rc = run_elem(c, pio);
-  FH_HIP(hipEventRecord(e1, io.stream));
-  FH_HIP(hipEventSynchronize(e1));
-  float plain_ms = 0.f;
-  FH_HIP(hipEventElapsedTime(&plain_ms, e0, e1));
+  float atomic_ms = 0.f, colour_ms = 0.f;
+  rc = time_it([&] { return run_elem(c, pio); }, &atomic_ms);
+  if (!rc) rc = time_it([&] { return run_elem(c, cio); }, &colour_ms);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  plain_ms /= reps;
-  if (plain_ms < chunk_ms) {
-    record_plain(c, dtype, op, batch, io.flags);
+  int path = PATH_CHUNK;
+  float best = chunk_ms;
+  if (atomic_ms < best) path = PATH_ATOMIC, best = atomic_ms;
+  if (colour_ms < best) path = PATH_COLOUR, best = colour_ms;
+  record_path(c, dtype, op, batch, io.flags, path);
+  if (path != PATH_CHUNK) {
     if (epc) *epc = 0;
-    if (block) *block = 0;
+    if (block) *block = path == PATH_COLOUR ? FEMHOT_PATH_COLOUR : FEMHOT_PATH_ATOMIC;
     if (grid_per_cu) *grid_per_cu = 0;
-    chunk_ms = plain_ms;
   }
-  if (ms) *ms = chunk_ms;
+  if (ms) *ms = best;
   return run_elem(c, io);  // leave valid outputs
 }
 
 int femhot_set_chunk_config(femhot_ctx *h, int op, int dtype, int batch, unsigned flags, int epc, int block,
                             int grid_per_cu) {
   FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
-  FH_CHECK_ARG(op >= FEMHOT_OP_RESIDUAL && op <= FEMHOT_OP_ENERGY_GRAD, FEMHOT_ERR_ARG, "bad op");
+  FH_CHECK_ARG(op >= FEMHOT_OP_RESIDUAL && op <= FEMHOT_OP_TANGENT, FEMHOT_ERR_ARG, "bad op");
   FH_CHECK_ARG(dtype == FEMHOT_F32 || dtype == FEMHOT_F64, FEMHOT_ERR_ARG, "bad dtype");
   return set_chunk_config(&h->c, op, dtype, batch, flags, epc, block, grid_per_cu);
 }

@@ -20,15 +20,23 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
 
 int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int epc, int block, int grid_per_cu);
 
+
 void forget_tuning(const void *ctx);
 
-// True when a call of this shape should take the plain path even though FEMHOT_SCATTER_CHUNK is set:
-// the autotuner found the plain path faster, or, untuned, the call is FP64 Neo-Hookean (its element
-// routine needs most of the register file, so only small chunks with small workgroups keep the chunk
-// kernel ahead; femhot_autotune times both and records the winner).
-bool chunk_prefers_plain(const Ctx *c, int dtype, int mode, int batch, unsigned flags);
+// The scatter a call takes when FEMHOT_SCATTER_CHUNK leaves the choice to the library: the LDS-staged
+// chunk kernel, the element-parallel kernel with hardware atomics, or the element-parallel kernel one
+// colour class at a time with plain read-modify-writes.
+enum ScatterPath { PATH_CHUNK = 0, PATH_ATOMIC = 1, PATH_COLOUR = 2 };
+// mode of the tangent assembly in the tuning table (next to MODE_RES / MODE_VJP / MODE_EGRAD)
+constexpr int MODE_TANGENT = 3;
 
-// Records that the plain path won the autotuner for this call shape.
-void record_plain(const Ctx *c, int dtype, int mode, int batch, unsigned flags);
+// The path femhot_autotune measured fastest for this call shape; untuned: the chunk kernel, except
+// FP64 Neo-Hookean (its element routine needs most of the register file, so only small chunks with
+// small workgroups keep the chunk kernel ahead) which takes the atomic path, and the tangent, which
+// takes the colour-ordered path.
+int tuned_path(const Ctx *c, int dtype, int mode, int batch, unsigned flags);
+
+// Records the path that won the autotuner for this call shape.
+void record_path(const Ctx *c, int dtype, int mode, int batch, unsigned flags, int path);
 
 }  // namespace femhot

@@ -124,8 +124,8 @@ __global__ void k_zero_nodes(const int *nodes, long long n, int P, int batch, R 
 // chunk's plan once and loops over sb samples); 0 picks enough items for about 8 per CU.
 struct ChunkConfig {
   int epc = 1024, block = 256, grid_per_cu = 0, sb = 0;
-  bool plain = false;  // the plain path won the autotuner for this call shape
-  bool dma = false;    // FP32 only: the pipelined LDS-DMA kernel (k_chunk_pipe) instead of k_chunk
+  int path = PATH_CHUNK;  // PATH_ATOMIC / PATH_COLOUR: an element-parallel path won the autotuner
+  bool dma = false;       // FP32 only: the pipelined LDS-DMA kernel (k_chunk_pipe) instead of k_chunk
 };
 
 static std::mutex g_tune_mu;
@@ -155,12 +155,26 @@ static size_t lds_limit() {
   return lim;
 }
 
+// fixed_ws: caller-provided scratch of fixed_bytes (the allocation-free entry point); then the plan
+// must already be built, nothing is allocated and no timing events are recorded.
 static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *field, long long fs, const void *u,
                          long long us, const void *w, long long ws, void *ou, long long ous, void *od, long long ods,
-                         double *energy, unsigned flags, hipStream_t s, const ChunkConfig &cfg) {
-  int rc = ensure_chunks(c, cfg.epc);
-  if (rc) return rc;
-  const ChunkPlan &p = c->chunk;
+                         double *energy, unsigned flags, hipStream_t s, const ChunkConfig &cfg,
+                         void *fixed_ws = nullptr, size_t fixed_bytes = 0) {
+  const ChunkPlan *plan = nullptr;
+  int rc;
+  if (fixed_ws) {
+    plan = find_chunks(c, cfg.epc);
+    if (!plan) {
+      set_error("femhot_energy_grad_noalloc: the chunk plan of " + std::to_string(cfg.epc) +
+                " elements per chunk is not built; build it first with femhot_set_chunk_config or a call of "
+                "femhot_energy_grad with FEMHOT_SCATTER_CHUNK");
+      return FEMHOT_ERR_ARG;
+    }
+  } else if ((rc = ensure_chunks(c, cfg.epc, s, &plan))) {
+    return rc;
+  }
+  const ChunkPlan &p = *plan;
   const bool elemental = (flags & FEMHOT_ELEMENT_FIELD) != 0;
   const int P = c->ndpn;
   const size_t rs = dtype == FEMHOT_F32 ? sizeof(float) : sizeof(double);
@@ -188,7 +202,13 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   const size_t pd_bytes = want_gd_nodal ? ((size_t)batch * p.n_slots * rs + 255) / 256 * 256 : 0;
   const size_t ce_bytes = energy ? ((size_t)batch * p.n_chunks * sizeof(double) + 255) / 256 * 256 : 0;
   const size_t need = pu_bytes + pd_bytes + ce_bytes + 256;
-  if (c->chunk_ws.bytes < need) {
+  if (fixed_ws && need > fixed_bytes) {
+    set_error("femhot_energy_grad_noalloc: this call needs " + std::to_string(need) + " bytes of scratch (shared-node " +
+              "partials and per-chunk energies), more than the library's fixed " + std::to_string(fixed_bytes) +
+              " bytes; use femhot_energy_grad, or fewer samples per call");
+    return FEMHOT_ERR_SHAPE;
+  }
+  if (!fixed_ws && c->chunk_ws.bytes < need) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     FH_HIP(hipStreamIsCapturing(s, &cs));
     if (cs != hipStreamCaptureStatusNone) {
@@ -202,7 +222,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
     FH_HIP(hipMalloc(&c->chunk_ws.ptr, need));
     c->chunk_ws.bytes = need;
   }
-  char *wsp = (char *)c->chunk_ws.ptr;
+  char *wsp = fixed_ws ? (char *)fixed_ws : (char *)c->chunk_ws.ptr;
   void *part_u = pu_bytes ? (void *)wsp : nullptr;
   void *part_d = pd_bytes ? (void *)(wsp + pu_bytes) : nullptr;
   double *chunk_energy = ce_bytes ? (double *)(wsp + pu_bytes + pd_bytes) : nullptr;
@@ -262,7 +282,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   if (cfg.grid_per_cu > 0) grid = std::min<long long>(items, (long long)cfg.grid_per_cu * cu_count());
   if (cfg.grid_per_cu < 0) grid = std::min<long long>(items, -(long long)cfg.grid_per_cu);
   if (grid < 1) return FEMHOT_OK;
-  if ((rc = timing_mark(c, s))) return rc;
+  if (!fixed_ws && (rc = timing_mark(c, s))) return rc;
   const dim3 gd((unsigned)grid);
   if (cfg.dma) {
     switch (c->et) {
@@ -323,7 +343,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
       if (rc) return rc;
     }
   }
-  return timing_mark(c, s);  // the timed region covers the chunk kernel and its reductions
+  return fixed_ws ? FEMHOT_OK : timing_mark(c, s);  // the timed region: chunk kernel and its reductions
 }
 
 // Default chunk size: the largest power of two <= 1024 whose per-corner LDS buffer stays within
@@ -427,7 +447,7 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[{c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)}] = best;
   }
-  int rc = ensure_chunks(c, best.epc);  // leave the winning plan resident
+  int rc = ensure_chunks(c, best.epc, nullptr, nullptr);  // leave the winning plan resident
   if (rc) return rc;
   if (best_epc) *best_epc = best.epc;
   if (best_block) *best_block = best.block | (best.dma ? (int)FEMHOT_CHUNK_DMA : 0);
@@ -437,6 +457,15 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
 }
 
 int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int epc, int block, int grid_per_cu) {
+  if (epc == 0 || mode == MODE_TANGENT) {  // an element-parallel path, as femhot_autotune reports it
+    if (block != FEMHOT_PATH_ATOMIC && block != FEMHOT_PATH_COLOUR) {
+      set_error("femhot_set_chunk_config: epc 0 (or the tangent) takes block FEMHOT_PATH_ATOMIC or FEMHOT_PATH_COLOUR");
+      return FEMHOT_ERR_ARG;
+    }
+    record_path(c, dtype, mode, mode == MODE_TANGENT ? 1 : batch, flags,
+                block == FEMHOT_PATH_COLOUR ? PATH_COLOUR : PATH_ATOMIC);
+    return FEMHOT_OK;
+  }
   const bool dma = (block & (int)FEMHOT_CHUNK_DMA) != 0;
   block &= ~(int)FEMHOT_CHUNK_DMA;
   if (epc < 1 || epc > 8192 || (block != 128 && block != 256 && block != 512) ||
@@ -454,23 +483,27 @@ int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned[{c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)}] = cfg;
   }
-  return ensure_chunks(c, epc);
+  return ensure_chunks(c, epc, nullptr, nullptr);
 }
 
-bool chunk_prefers_plain(const Ctx *c, int dtype, int mode, int batch, unsigned flags) {
+int tuned_path(const Ctx *c, int dtype, int mode, int batch, unsigned flags) {
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find({c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)});
-    if (it != g_tuned.end()) return it->second.plain;
+    if (it != g_tuned.end()) return it->second.path;
   }
-  return dtype == FEMHOT_F64 && c->ph == NEOHOOKE;
+  if (mode == MODE_TANGENT) return PATH_COLOUR;
+  return (dtype == FEMHOT_F64 && c->ph == NEOHOOKE) ? PATH_ATOMIC : PATH_CHUNK;
 }
 
-void record_plain(const Ctx *c, int dtype, int mode, int batch, unsigned flags) {
+void record_path(const Ctx *c, int dtype, int mode, int batch, unsigned flags, int path) {
   ChunkConfig cfg;
-  cfg.plain = true;
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tuned[{c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)}] = cfg;
+  auto key = std::make_tuple((const void *)c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD));
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) cfg = it->second;  // keeps the best chunk configuration beside the path
+  cfg.path = path;
+  g_tuned[key] = cfg;
 }
 
 void forget_tuning(const void *ctx) {

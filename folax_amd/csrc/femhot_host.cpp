@@ -235,13 +235,7 @@ static double chunk_plan_cost(const Ctx *c, std::vector<int> &order, int epc, co
   return bytes;
 }
 
-int ensure_chunks(Ctx *c, int epc) {
-  if (c->chunk.n_chunks && c->chunk.epc == epc) return FEMHOT_OK;
-  if (epc < 1 || epc > 8192) {
-    set_error("ensure_chunks: elements per chunk must be in [1, 8192]");
-    return FEMHOT_ERR_ARG;
-  }
-  free_chunks(c->chunk);
+static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
   const int n = c->n, D = c->dim;
   const long long Ne = c->Ne;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -283,8 +277,8 @@ int ensure_chunks(Ctx *c, int epc) {
     for (long long e = 0; e < Ne; ++e) morton[e] = key[e].second, ident[e] = (int)e;
     const double cost_m = chunk_plan_cost(c, morton, epc, seg_of, false);
     const double cost_i = chunk_plan_cost(c, ident, epc, seg_of, true);
-    c->chunk.perm_identity = cost_i <= cost_m;
-    perm = c->chunk.perm_identity ? ident : morton;
+    p.perm_identity = cost_i <= cost_m;
+    perm = p.perm_identity ? ident : morton;
   }
   long long i = 0;
   std::vector<int> loc, loc2, lid, icount, lorder, lrank;
@@ -384,7 +378,6 @@ int ensure_chunks(Ctx *c, int epc) {
   std::vector<int> seg_chunk_off(c->nseg + 1, 0);
   for (int s : seg) seg_chunk_off[s + 1]++;
   for (int s = 0; s < c->nseg; ++s) seg_chunk_off[s + 1] += seg_chunk_off[s];
-  ChunkPlan &p = c->chunk;
   p.epc = epc;
   p.n_chunks = (int)seg.size();
   p.max_nodes = max_nodes;
@@ -408,6 +401,64 @@ int ensure_chunks(Ctx *c, int epc) {
     return rc;
   }
   return FEMHOT_OK;
+}
+
+// Plans are cached per elements-per-chunk, so call shapes tuned to different chunk sizes (a forward
+// and its VJP, FP32 and FP64 calls) alternate without rebuilding.  A plan is built on the host and
+// uploaded with blocking copies, so it is never built while `s` is being captured into a graph.
+int ensure_chunks(Ctx *c, int epc, hipStream_t s, const ChunkPlan **out) {
+  for (ChunkPlan &p : c->plans)
+    if (p.epc == epc && p.n_chunks) {
+      p.last_use = ++c->plan_tick;
+      if (out) *out = &p;
+      return FEMHOT_OK;
+    }
+  if (epc < 1 || epc > 8192) {
+    set_error("ensure_chunks: elements per chunk must be in [1, 8192]");
+    return FEMHOT_ERR_ARG;
+  }
+  if (s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FH_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+      set_error("ensure_chunks: the chunk plan of " + std::to_string(epc) +
+                " elements per chunk is not built; build it before capture with an uncaptured call of the same "
+                "shape or femhot_set_chunk_config");
+      return FEMHOT_ERR_ARG;
+    }
+  }
+  if (c->plans.size() >= kMaxChunkPlans) {  // evict the least recently used plan
+    auto victim = c->plans.begin();
+    for (auto it = c->plans.begin(); it != c->plans.end(); ++it)
+      if (it->last_use < victim->last_use) victim = it;
+    FH_HIP(hipDeviceSynchronize());
+    free_chunks(*victim);
+    c->plans.erase(victim);
+  }
+  c->plans.emplace_back();
+  ChunkPlan &p = c->plans.back();
+  const int rc = build_chunk_plan(c, epc, p);
+  if (rc) {
+    c->plans.pop_back();
+    return rc;
+  }
+  p.last_use = ++c->plan_tick;
+  if (out) *out = &p;
+  return FEMHOT_OK;
+}
+
+const ChunkPlan *find_chunks(Ctx *c, int epc) {
+  for (ChunkPlan &p : c->plans)
+    if (p.epc == epc && p.n_chunks) {
+      p.last_use = ++c->plan_tick;
+      return &p;
+    }
+  return nullptr;
+}
+
+void free_chunk_plans(Ctx *c) {
+  for (ChunkPlan &p : c->plans) free_chunks(p);
+  c->plans.clear();
 }
 
 int timing_mark(Ctx *c, hipStream_t s) {
@@ -506,6 +557,15 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
       set_error("femhot_ctx_create: connectivity index out of range");
       return FEMHOT_ERR_INDEX;
     }
+  // collapsed elements (a node listed twice) are valid input, but two lanes of one element would
+  // then update the same tangent entries: the colour-ordered tangent falls back to atomics for them
+  for (long long e = 0; e < c.Ne && !c.repeated_nodes; ++e)
+    for (int a = 0; a < c.n && !c.repeated_nodes; ++a)
+      for (int b = a + 1; b < c.n; ++b)
+        if (c.h_conn[e * c.n + a] == c.h_conn[e * c.n + b]) {
+          c.repeated_nodes = true;
+          break;
+        }
   for (long long i = 0; i < c.Nn * 3; ++i)
     if (!std::isfinite(c.h_coords[i])) {
       delete h;
@@ -600,7 +660,7 @@ void femhot_ctx_destroy(femhot_ctx *h) {
   (void)hipFree(c.d_col_idx);
   (void)hipFree(c.d_slot);
   (void)hipFree(c.d_colour_elems);
-  free_chunks(c.chunk);
+  free_chunk_plans(&c);
   (void)hipFree(c.chunk_ws.ptr);
   (void)hipFree(c.newton_ws.ptr);
   (void)hipFree(c.d_ws);

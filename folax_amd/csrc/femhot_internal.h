@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <list>
 #include <string>
 #include <vector>
 
@@ -62,7 +63,11 @@ struct ChunkPlan {
   // entries touching it, sorted by element position; per chunk the list has n * elements entries
   uint16_t *inc = nullptr;        // (Ne * n), chunk by chunk
   uint16_t *inc_start = nullptr;  // (chunk nodes) start of each node's entries within its chunk
+  unsigned long long last_use = 0;  // plan cache: least recently used plan is evicted first
 };
+
+// Chunk plans kept per context (one per elements-per-chunk in use).
+constexpr size_t kMaxChunkPlans = 4;
 
 // Device scratch of the chunk path (partials and per-chunk energies), grown on demand.
 struct ChunkWork {
@@ -74,6 +79,7 @@ struct Ctx {
   femhot_mesh_desc desc{};
   int et = 0, ph = 0, ngp = 0, n = 0, dim = 0, ndpn = 0;
   long long Nn = 0, Ne = 0, Ndof = 0;
+  bool repeated_nodes = false;  // some element lists a node twice (collapsed element)
   PhysParams pp{};
   int nseg = 1;
   // host copies
@@ -101,7 +107,8 @@ struct Ctx {
   int n_colours = 0;
   std::vector<int> h_colour_off;  // (n_colours+1)
   int *d_colour_elems = nullptr;  // elements sorted by colour (stable)
-  ChunkPlan chunk;                // built on demand
+  std::list<ChunkPlan> plans;     // built on demand, cached per elements-per-chunk (stable addresses)
+  unsigned long long plan_tick = 0;
   ChunkWork chunk_ws;
   ChunkWork newton_ws;            // femhot_newton: tangent values, residual, rhs, step, Dirichlet values
   bool newton_dval_ready = false;
@@ -117,7 +124,12 @@ struct Ctx {
 
 int ensure_pattern(Ctx *c);
 int ensure_colouring(Ctx *c);
-int ensure_chunks(Ctx *c, int epc);
+// The chunk plan of `epc` elements per chunk, built (and cached) on first use; refused while `s`
+// is being captured into a graph.
+int ensure_chunks(Ctx *c, int epc, hipStream_t s, const ChunkPlan **out);
+// The cached plan of `epc` elements per chunk, or null when it is not built (never builds).
+const ChunkPlan *find_chunks(Ctx *c, int epc);
+void free_chunk_plans(Ctx *c);
 int ensure_workspace(Ctx *c, size_t bytes);
 
 // Records an event on `s` when timing is enabled (pairs: before/after the dominant kernel).

@@ -127,3 +127,89 @@ extern "C" int femhot_residual_shape_vjp(femhot_ctx *h, int dtype, const void *f
     default: return launch_shape<TET4>(dtype, c->ph, c->ngp, grid, s, A);
   }
 }
+
+// ---------------------------------------------------------------------------------------------
+// Gauss-point geometry of every element: the integration data a response integrand needs
+// (fol/responses/fe_response.py:92-113 evaluates its formula at each Gauss point with weight w detJ,
+// field N.d and state N.u; its shape gradient needs d(detJ)/dX = detJ dN/dx).
+// ---------------------------------------------------------------------------------------------
+namespace femhot {
+
+struct GPDataArgs {
+  const double *coords;
+  const int *conn;
+  long long Ne;
+  void *N, *wdet, *dndx;
+};
+
+template <class R, int ET, int NGP>
+__global__ __launch_bounds__(128) void k_gp_data(GPDataArgs A) {
+  constexpr int N = ElemT<ET>::N, D = ElemT<ET>::D, G = RuleSize<ET, NGP>::G;
+  const long long e = (long long)blockIdx.x * 128 + threadIdx.x;
+  if (e >= A.Ne) return;
+  int nodes[N];
+  load_conn<N>(A.conn, e, nodes);
+  R x[N][D];
+  double x0[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) x0[i] = A.coords[(size_t)nodes[0] * 3 + i];
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int i = 0; i < D; ++i) x[a][i] = R(A.coords[(size_t)nodes[a] * 3 + i] - x0[i]);
+  R *Wd = reinterpret_cast<R *>(A.wdet) + e * G;
+  R *Dn = reinterpret_cast<R *>(A.dndx) + e * G * N * D;
+  R *Nt = reinterpret_cast<R *>(A.N);
+#pragma unroll 1
+  for (int g = 0; g < G; ++g) {
+    GP<R, ET> gp;
+    gp_geometry<R, ET, NGP>(g, x, gp);
+    if (Wd) Wd[g] = gp.wdet;
+    if (Dn)
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+#pragma unroll
+        for (int j = 0; j < D; ++j) Dn[(g * N + a) * D + j] = gp.DN[a][j];
+    if (Nt && e == 0)
+#pragma unroll
+      for (int a = 0; a < N; ++a) Nt[g * N + a] = gp.Nv[a];
+  }
+}
+
+template <int ET> static int launch_gp_data(int dtype, int ngp, dim3 grid, hipStream_t s, const GPDataArgs &A) {
+  auto go = [&](auto rp) -> int {
+    using R = std::remove_pointer_t<decltype(rp)>;
+    if (ngp == 1)
+      hipLaunchKernelGGL((k_gp_data<R, ET, 1>), grid, dim3(128), 0, s, A);
+    else if (ngp == 2)
+      hipLaunchKernelGGL((k_gp_data<R, ET, 2>), grid, dim3(128), 0, s, A);
+    else
+      hipLaunchKernelGGL((k_gp_data<R, ET, 3>), grid, dim3(128), 0, s, A);
+    FH_HIP(hipGetLastError());
+    return FEMHOT_OK;
+  };
+  return dtype == FEMHOT_F32 ? go((float *)nullptr) : go((double *)nullptr);
+}
+
+}  // namespace femhot
+
+extern "C" int femhot_gauss_point_data(femhot_ctx *h, int dtype, void *N, void *wdetJ, void *dNdx, void *stream) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  FH_CHECK_ARG(dtype == FEMHOT_F32 || dtype == FEMHOT_F64, FEMHOT_ERR_ARG, "dtype must be FEMHOT_F32 or FEMHOT_F64");
+  FH_CHECK_ARG(N || wdetJ || dNdx, FEMHOT_ERR_ARG, "no output requested");
+  Ctx *c = &h->c;
+  GPDataArgs A{};
+  A.coords = c->d_coords64;
+  A.conn = c->d_conn;
+  A.Ne = c->Ne;
+  A.N = N;
+  A.wdet = wdetJ;
+  A.dndx = dNdx;
+  const dim3 grid((unsigned)((c->Ne + 127) / 128));
+  hipStream_t s = (hipStream_t)stream;
+  switch (c->et) {
+    case QUAD4: return launch_gp_data<QUAD4>(dtype, c->ngp, grid, s, A);
+    case HEX8: return launch_gp_data<HEX8>(dtype, c->ngp, grid, s, A);
+    default: return launch_gp_data<TET4>(dtype, c->ngp, grid, s, A);
+  }
+}

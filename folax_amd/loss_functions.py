@@ -95,8 +95,20 @@ class FiniteElementLoss:
         self.num_gp = num_gp
         self.loss_settings["num_gp"] = num_gp
         self.dim = self.loss_settings["compute_dims"]
-        if self.loss_settings.get("parametric_boundary_learning"):
-            raise NotImplementedError("parametric_boundary_learning is not supported by this backend")
+        # fe_loss.py:91-103: with parametric_boundary_learning the Dirichlet dofs of a sample are
+        # its own known_dofs (the batch parameters) instead of the fixed values, and the control
+        # field comes from get_param_function (identity in the base class; a constant or mesh
+        # point-data heterogeneity field when material_dict names one, as
+        # mechanical_saint_venant.py:59-66 does)
+        self.parametric_boundary_learning = bool(self.loss_settings.get("parametric_boundary_learning"))
+        self.get_param_function = lambda x: x
+        md = self.loss_settings.get("material_dict", {})
+        if self.parametric_boundary_learning and "heterogeneity_field_name" in md:
+            name = md["heterogeneity_field_name"]
+            if not self.fe_mesh.HasPointData(name):
+                self.fe_mesh[name] = md["heterogeneity_default_value"] * np.ones(self.fe_mesh.GetNumberOfNodes())
+            self.heterogeneity_field = np.asarray(self.fe_mesh[name], dtype=np.float64)
+            self.get_param_function = self._heterogeneity_params
         self.loss_function_exponent = float(self.loss_settings.get("loss_function_exponent", 1.0))
         self.control_location = self.loss_settings.get("control_location", "node")
         # "chunk": LDS-staged, atomic-free and deterministic (the library still takes the plain path
@@ -147,13 +159,23 @@ class FiniteElementLoss:
     def GetDOFs(self):
         return self.dofs
 
-    # ---- fe_loss.py:91-100, 124-125, 186-189 ----
+    # ---- fe_loss.py:91-100, 124-128, 186-189 ----
     def GetFullDofVector(self, known_dofs, unknown_dofs):
+        """Dirichlet dofs overwritten with the fixed values, or, with parametric_boundary_learning,
+        with known_dofs ((B, n_dirichlet) or (n_dirichlet,), in dirichlet_indices order)."""
         u = _as_device_tensor(unknown_dofs)
         idx, val = self._dir_idx(u.device)
         out = u.clone()
-        out[..., idx] = val.to(u.dtype)
+        if self.parametric_boundary_learning:
+            k = _as_device_tensor(known_dofs, dtype=u.dtype, device=u.device)
+            out[..., idx] = k.reshape(*out.shape[:-1], idx.numel()) if out.dim() > 1 else k.reshape(-1)
+        else:
+            out[..., idx] = val.to(u.dtype)
         return out
+
+    def _heterogeneity_params(self, param_vector):
+        p = torch.as_tensor(self.heterogeneity_field, device=param_vector.device, dtype=param_vector.dtype)
+        return p.expand(param_vector.shape[0], -1) if param_vector.dim() > 1 else p
 
     def ApplyDirichletBCOnDofVector(self, full_dof_vector, load_increment: float = 1.0):
         u = _as_device_tensor(full_dof_vector)
@@ -163,7 +185,7 @@ class FiniteElementLoss:
         return out
 
     def GetParametersVectors(self, param_vector):
-        return param_vector
+        return self.get_param_function(param_vector)
 
     # ---- energies ----
     def ComputeTotalEnergy(self, total_control_vars, total_primal_vars):
@@ -179,8 +201,8 @@ class FiniteElementLoss:
         bp = batch_params if isinstance(batch_params, torch.Tensor) else _as_device_tensor(batch_params, dtype=bd.dtype)
         bp = torch.atleast_2d(bp).reshape(bp.shape[0] if bp.dim() > 1 else 1, -1)
         bd = torch.atleast_2d(bd).reshape(bd.shape[0] if bd.dim() > 1 else 1, -1)
-        full = _FullDofs.apply(self, bd)
-        energies = _BatchEnergy.apply(self, bp, full)
+        full = _FullDofs.apply(self, bp, bd)
+        energies = _BatchEnergy.apply(self, torch.atleast_2d(self.GetParametersVectors(bp)), full)
         be = energies ** self.loss_function_exponent
         m = be.mean()
         return m, (be.min(), be.max(), m)
@@ -191,9 +213,12 @@ class FiniteElementLoss:
         the supported physics are symmetric, so transpose_jacobian leaves them unchanged."""
         u = _as_device_tensor(total_primal_vars)
         d = _as_device_tensor(total_control_vars, dtype=u.dtype)
+        # "chunk" leaves the assembly order to the library: the path femhot_autotune measured for the
+        # tangent (with autotune on), else colour-ordered (deterministic); "atomic": hardware atomics
         flags = self._flags() | (L.TRANSPOSE if transpose_jacobian else 0)
-        if self.scatter == "chunk":  # colour-ordered assembly: deterministic, and faster than atomics
-            flags |= L.SCATTER_COLOUR
+        if self.scatter == "chunk" and self.autotune and (L.OP_TANGENT, u.dtype, 1) not in self._tuned:
+            self.ctx.autotune_tangent(d, u, flags=flags & ~L.SCATTER_CHUNK)
+            self._tuned.add((L.OP_TANGENT, u.dtype, 1))
         vals, R = self.ctx.tangent_csr(d, u, want_R=True, flags=flags)
         if not hasattr(self, "_pattern") or self._pattern[0].device != u.device:
             self._pattern = self.ctx.pattern(u.device)
@@ -239,24 +264,30 @@ class FiniteElementLoss:
         with respect to both arguments through the residual VJP kernel (exact unless picard)."""
         bd = batch_dofs if isinstance(batch_dofs, torch.Tensor) else _as_device_tensor(batch_dofs)
         bp = batch_params if isinstance(batch_params, torch.Tensor) else _as_device_tensor(batch_params, dtype=bd.dtype)
-        full = _FullDofs.apply(self, torch.atleast_2d(bd))
-        return _ResidualNormLoss.apply(self, torch.atleast_2d(bp), full, picard)
+        bp2 = torch.atleast_2d(bp)
+        full = _FullDofs.apply(self, bp2, torch.atleast_2d(bd))
+        return _ResidualNormLoss.apply(self, torch.atleast_2d(self.GetParametersVectors(bp2)), full, picard)
 
 
 class _FullDofs(torch.autograd.Function):
-    """GetFullDofVector: Dirichlet entries overwritten; their gradient is zero."""
+    """GetFullDofVector: Dirichlet entries overwritten, so their gradient with respect to the
+    incoming dofs is zero; with parametric_boundary_learning they are the known dofs, which receive
+    the gradient of the entries they set (a gather of g at dirichlet_indices, as the transpose of
+    the reference's .at[:, dirichlet_indices].set(known_dofs))."""
 
     @staticmethod
-    def forward(ctx, loss, dofs):
+    def forward(ctx, loss, known, dofs):
         ctx.loss = loss
-        return loss.GetFullDofVector(None, dofs)
+        return loss.GetFullDofVector(known, dofs)
 
     @staticmethod
     def backward(ctx, g):
-        idx, _ = ctx.loss._dir_idx(g.device)
+        loss = ctx.loss
+        idx, _ = loss._dir_idx(g.device)
+        gk = g[..., idx].clone() if loss.parametric_boundary_learning else None
         g = g.clone()
         g[..., idx] = 0
-        return None, g
+        return None, gk, g
 
 
 class _BatchEnergy(torch.autograd.Function):

@@ -140,6 +140,13 @@ class Mesh:
     def __setitem__(self, key, value):
         self.point_data[key] = np.asarray(value)
 
+    def __getitem__(self, key):
+        return self.point_data[key]
+
+    def HasPointData(self, data_name):
+        """mesh.py:174-175."""
+        return data_name in self.point_data
+
     def Finalize(self, export_dir: str = ".", export_format: str = "vtk"):
         """Writes the mesh and the assigned point data (mesh.py:177-180); legacy VTK only."""
         import os

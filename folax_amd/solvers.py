@@ -66,6 +66,20 @@ class FiniteElementLinearResidualBasedSolver(FiniteElementSolver):
         return u + self.LinearSolve(K, R, u)
 
 
+class AdjointFiniteElementSolver(FiniteElementSolver):
+    """fol/solvers/adjoint_fe_solver.py:11-23: solves K^T lambda = -df/du with the response's
+    BC-applied transposed tangent (device CSR assembly + PCG)."""
+
+    def __init__(self, adj_fe_solver_name, fe_response, adj_fe_solver_settings=None):
+        super().__init__(adj_fe_solver_name, fe_response.fe_loss, adj_fe_solver_settings or {})
+        self.fe_response = fe_response
+
+    def Solve(self, current_control_vars, current_dofs, current_adjoint_dofs=None):
+        jac, rhs = self.fe_response.ComputeAdjointJacobianMatrixAndRHSVector(current_control_vars, current_dofs)
+        # LinearSolve solves K x = -r, and rhs already carries the minus sign of -df/du
+        return self.LinearSolve(jac, -rhs, current_adjoint_dofs)
+
+
 class FiniteElementNonLinearResidualBasedSolver(FiniteElementLinearResidualBasedSolver):
     def __init__(self, fe_solver_name, fe_loss_function, fe_solver_settings=None, history_plot_settings=None):
         super().__init__(fe_solver_name, fe_loss_function, fe_solver_settings)
@@ -84,9 +98,9 @@ class FiniteElementNonLinearResidualBasedSolver(FiniteElementLinearResidualBased
         u = _as_device_tensor(current_dofs_np).to(torch.float64)
         d = _as_device_tensor(current_control_vars, dtype=torch.float64)
         s = self.nonlinear_solver_settings
-        flags = loss._flags() & L.ELEMENT_FIELD
-        if loss.scatter == "chunk":  # colour-ordered assembly, as ComputeJacobianMatrixAndResidualVector
-            flags |= L.SCATTER_COLOUR
+        # the tangent path of ComputeJacobianMatrixAndResidualVector: the library's tuned choice under
+        # "chunk" (colour-ordered untuned), hardware atomics under "atomic"
+        flags = loss._flags() & (L.ELEMENT_FIELD | L.SCATTER_CHUNK)
         try:
             u, hist, steps = loss.ctx.newton(d, u, load_steps=s["load_incr"], max_iter=s["maxiter"],
                                              abs_tol=s["abs_tol"], rel_tol=s["rel_tol"],

@@ -139,6 +139,15 @@ int femhot_residual_vjp(femhot_ctx *ctx, int dtype, int batch, const void *field
 int femhot_residual_shape_vjp(femhot_ctx *ctx, int dtype, const void *field, const void *u, const void *w,
                               void *g_x, unsigned flags, void *stream);
 
+/* Integration data of every element at the context's Gauss rule (G points per element: QUAD4 g^2,
+ * HEX8 g^3, TET4 1/4/8 for rule g = 1/2/3), DEVICE outputs in the arithmetic type, each optional:
+ * N (G, nodes_per_elem) shape function values; wdetJ (n_elems, G) weight x signed detJ;
+ * dNdx (n_elems, G, nodes_per_elem, dim) global shape gradients.  Replaces the per-Gauss-point
+ * geometry that FiniteElementResponse.ComputeResponseElementValue evaluates before its response
+ * formula (fol/responses/fe_response.py:92-113): the response integrand and its derivatives are
+ * formed from these (d detJ / dX_a = detJ dN_a/dx gives the integrand's shape gradient). */
+int femhot_gauss_point_data(femhot_ctx *ctx, int dtype, void *N, void *wdetJ, void *dNdx, void *stream);
+
 /* Potential Pi and its gradient: g_u = dPi/du (= residual), g_field = dPi/dfield.
  * energy: double (batch * max(1,n_segments)) or NULL. */
 int femhot_energy_grad(femhot_ctx *ctx, int dtype, int batch, const void *field, long long field_stride,
@@ -168,7 +177,8 @@ int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *c
  * else u += du (:135-159).  All vectors stay on the device; the host waits twice per iteration
  * (the two norms) besides the linear solver's own convergence checks.
  * field: FP64 DEVICE, (n_nodes) or (n_elems) with FEMHOT_ELEMENT_FIELD.  u: FP64 DEVICE (n_dofs),
- * initial guess in, solution out.  flags: FEMHOT_ELEMENT_FIELD | FEMHOT_SCATTER_COLOUR only.
+ * initial guess in, solution out.  flags: FEMHOT_ELEMENT_FIELD | FEMHOT_SCATTER_COLOUR |
+ * FEMHOT_SCATTER_CHUNK only (the last: the tangent path femhot_autotune measured, else colour-ordered).
  * history: HOST (load_steps x max_iter x 2) doubles or NULL: (||R||, ||du||) per iteration, NaN
  * after a load step's last iteration.  steps: HOST (load_steps) ints or NULL: iterations taken per
  * load step.  Synchronises the stream. */
@@ -176,21 +186,30 @@ int femhot_newton(femhot_ctx *ctx, int mode, const double *field, double *u, uns
                   int max_iter, double abs_tol, double rel_tol, double lin_tol, int lin_maxiter, double *history,
                   int *steps, void *stream);
 
-/* Autotuner for the LDS-staged chunk path (FEMHOT_SCATTER_CHUNK): times elements-per-chunk x
- * workgroup size x grid shape (one workgroup per chunk, or a persistent grid of k workgroups per
- * CU) x samples per work item on these inputs, and the plain path beside them, and keeps the
- * fastest for (ctx, op, dtype, batch, field location); FEMHOT_SCATTER_CHUNK calls of that shape then
- * use it.  *epc = 0 reports that the plain path won.  Untuned, FEMHOT_SCATTER_CHUNK calls take the
- * chunk path except in FP64 Neo-Hookean, which takes the plain path.  Finally runs the operation
- * once more so the outputs are valid.  Synchronises the stream (not for capture).
- * op: FEMHOT_OP_RESIDUAL / FEMHOT_OP_VJP / FEMHOT_OP_ENERGY_GRAD.  No counterpart in the
- * reference (XLA fuses the element vmap itself); it replaces fe_loss.py:105-114's fixed batch size.
- * In FP32 the candidates include the software-pipelined chunk kernel, whose staging is LDS DMA
- * issued one step ahead of the arithmetic; *block then carries FEMHOT_CHUNK_DMA. */
+/* Autotuner of the scatter path that FEMHOT_SCATTER_CHUNK leaves to the library: times
+ * elements-per-chunk x workgroup size x grid shape (one workgroup per chunk, or a persistent grid
+ * of k workgroups per CU) x samples per work item of the LDS-staged chunk kernel on these inputs,
+ * and beside them the element-parallel kernel with hardware atomics and the colour-ordered
+ * element-parallel kernel, and keeps the fastest for (ctx, op, dtype, batch, field location);
+ * FEMHOT_SCATTER_CHUNK calls of that shape then take it.  *epc = 0 reports that an
+ * element-parallel path won, *block then FEMHOT_PATH_ATOMIC or FEMHOT_PATH_COLOUR.  Untuned,
+ * FEMHOT_SCATTER_CHUNK calls take the chunk path except in FP64 Neo-Hookean, which takes the
+ * atomic path.  Finally runs the operation once more so the outputs are valid.
+ * op FEMHOT_OP_TANGENT times femhot_tangent_csr with atomics against colour-ordered assembly (out_u
+ * = the CSR values, no residual) and keeps the faster for femhot_tangent_csr calls with
+ * FEMHOT_SCATTER_CHUNK (untuned: colour-ordered).  Synchronises the stream (not for capture).
+ * op: FEMHOT_OP_RESIDUAL / FEMHOT_OP_VJP / FEMHOT_OP_ENERGY_GRAD / FEMHOT_OP_TANGENT.  No
+ * counterpart in the reference (XLA fuses the element vmap itself); it replaces fe_loss.py:105-114's
+ * fixed batch size.  In FP32 the candidates include the software-pipelined chunk kernel, whose
+ * staging is LDS DMA issued one step ahead of the arithmetic; *block then carries FEMHOT_CHUNK_DMA. */
 #define FEMHOT_CHUNK_DMA 0x10000
 #define FEMHOT_OP_RESIDUAL 0
 #define FEMHOT_OP_VJP 1
 #define FEMHOT_OP_ENERGY_GRAD 2
+#define FEMHOT_OP_TANGENT 3   /* femhot_tangent_csr: atomic against colour-ordered assembly; out_u = values */
+/* *block when an element-parallel path won (*epc = 0): hardware atomics, or colour-ordered plain adds */
+#define FEMHOT_PATH_ATOMIC 1
+#define FEMHOT_PATH_COLOUR 2
 int femhot_autotune(femhot_ctx *ctx, int op, int dtype, int batch, const void *field, long long field_stride,
                     const void *u, long long u_stride, const void *w, long long w_stride, void *out_u,
                     long long out_u_stride, void *out_field, long long out_field_stride, double *energy,
@@ -201,6 +220,8 @@ int femhot_autotune(femhot_ctx *ctx, int op, int dtype, int batch, const void *f
  * calls (and graph captures) allocate nothing.  block: 128 / 256 / 512, or 256 / 512 |
  * FEMHOT_CHUNK_DMA for the pipelined LDS-DMA kernel (FP32 only).  grid_per_cu: 0 = one workgroup
  * per work item, k > 0 = k persistent workgroups per CU, k < 0 = -k persistent workgroups in all.
+ * epc = 0 (and always for op FEMHOT_OP_TANGENT) records an element-parallel path instead, block =
+ * FEMHOT_PATH_ATOMIC or FEMHOT_PATH_COLOUR, as femhot_autotune reports it.
  * No counterpart in the reference. */
 int femhot_set_chunk_config(femhot_ctx *ctx, int op, int dtype, int batch, unsigned flags, int epc, int block,
                             int grid_per_cu);

@@ -35,68 +35,105 @@ def _sa_loss(N):
     return m, loss
 
 
-def _response_grads(coords, conn, K, u):
-    """Response (E**2)*U[0] (fe_response.py:73-113) gradients: test-side, not part of the hot path."""
-    N, w, DN, detJ = fe.geometry("quad", coords[conn], 2)
-    Eg = np.einsum("ga,ea->eg", N, K[conn])
-    dofs = fe.element_dofs(conn, 2)
-    U0 = np.einsum("ga,ea->eg", N, u[dofs][:, 0::2])
-    wd = w[None] * detJ
-    dfdu_e = np.zeros((conn.shape[0], 8))
-    dfdu_e[:, 0::2] = np.einsum("eg,ga->ea", wd * Eg ** 2, N)
-    dfdu = np.zeros(2 * coords.shape[0])
-    np.add.at(dfdu, dofs, dfdu_e)
-    dfdE = np.zeros(coords.shape[0])
-    np.add.at(dfdE, conn, np.einsum("eg,ga->ea", wd * 2 * Eg * U0, N))
-    return dfdu, dfdE
+def _response(loss, m):
+    """The reference tests' response (E**2)*U[0] over a Fourier control named E
+    (test_sensitivity_analysis.py:31-36, test_mechanical_2D_sa.py:46-51)."""
+    from folax_amd.controls import FourierControl
+    from folax_amd.responses import FiniteElementResponse
+    fc = FourierControl("E", {"x_freqs": np.array([2, 4, 6]), "y_freqs": np.array([2, 4, 6]),
+                              "z_freqs": np.array([0]), "beta": 20, "min": 1e-1, "max": 1}, m)
+    r = FiniteElementResponse("test_response", response_formula="(E**2)*U[0]", fe_loss=loss, control=fc)
+    r.Initialize()
+    return r
 
 
 def test_sensitivity_analysis_golden_through_api():
-    """tests/unit/test_sensitivity_analysis.py through the drop-in API."""
+    """tests/unit/test_sensitivity_analysis.py through FiniteElementResponse: the transposed BC
+    tangent row 8, the adjoint right-hand side, and the adjoint control and shape derivatives."""
     sa = G["tests/unit/test_sensitivity_analysis.py"]["test_quad"]
-    K = np.array(sa["assigns"]["random_K"])
-    u = np.array(sa["assigns"]["random_FE_UV"])
-    lam = np.array(sa["assigns"]["random_adj_FE_UV"])
+    K = t64(sa["assigns"]["random_K"])
+    u = t64(sa["assigns"]["random_FE_UV"])
+    lam = t64(sa["assigns"]["random_adj_FE_UV"])
     m, loss = _sa_loss(3)
-    A, _ = loss.ComputeJacobianMatrixAndResidualVector(t64(K), t64(u), transpose_jacobian=True)
-    chk = sa["checks"][0]
-    np.testing.assert_allclose(A.todense()[8].cpu().numpy(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
-    _, gd = loss.ComputeResidualVJP(t64(K), t64(u), t64(lam))
-    _, dfdE = _response_grads(m.GetNodesCoordinates(), m.GetElementsNodes("quad"), K, u)
-    chk = sa["checks"][2]
-    np.testing.assert_allclose(dfdE + gd.cpu().numpy(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
-    from tests.test_oracle_goldens import _response_shape_grad
-    gx = loss.ComputeResidualShapeVJP(t64(K), t64(u), t64(lam)).cpu().numpy()
-    chk = sa["checks"][3]
-    dfdX = _response_shape_grad(m.GetNodesCoordinates(), m.GetElementsNodes("quad"), K, u).ravel()
-    np.testing.assert_allclose(dfdX + gx, chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+    resp = _response(loss, m)
+    A, rhs = resp.ComputeAdjointJacobianMatrixAndRHSVector(K, u)
+    for chk, actual in zip(sa["checks"][:2], (A.todense()[8], rhs)):
+        np.testing.assert_allclose(actual.cpu().numpy().ravel(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+    gd = resp.ComputeAdjointNodalControlDerivatives(K, u, lam)
+    gx = resp.ComputeAdjointNodalShapeDerivatives(K, u, lam)
+    for chk, actual in zip(sa["checks"][2:4], (gd, gx)):
+        np.testing.assert_allclose(actual.cpu().numpy().ravel(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
 
 
 @pytest.mark.parametrize("solver", ["PCG", "PCG-mixed"])
 def test_mechanical_2D_sa_golden_through_api(solver):
-    """tests/integration/test_mechanical_2D_sa.py: linear solve -> adjoint solve -> control
-    derivatives, with the device PCG in place of the direct solver."""
-    from folax_amd.solvers import FiniteElementLinearResidualBasedSolver
+    """tests/integration/test_mechanical_2D_sa.py: linear solve -> AdjointFiniteElementSolver ->
+    adjoint control and shape derivatives, with the device PCG in place of the direct solver."""
+    from folax_amd.solvers import AdjointFiniteElementSolver, FiniteElementLinearResidualBasedSolver
     sa = G["tests/integration/test_mechanical_2D_sa.py"]
-    K = np.array(sa["setUp"]["assigns"]["self.random_K"])
+    K = t64(sa["setUp"]["assigns"]["self.random_K"])
     m, loss = _sa_loss(5)
-    s = FiniteElementLinearResidualBasedSolver("lin", loss, {"linear_solver_settings": {"solver": solver, "tol": 1e-12}})
+    resp = _response(loss, m)
+    settings = {"linear_solver_settings": {"solver": solver, "tol": 1e-12}}
+    s = FiniteElementLinearResidualBasedSolver("lin", loss, settings)
     s.Initialize()
-    u = s.Solve(t64(K), np.zeros(2 * m.GetNumberOfNodes()))
-    Kt, _ = loss.ComputeJacobianMatrixAndResidualVector(t64(K), u, transpose_jacobian=True)
-    coords, conn = m.GetNodesCoordinates(), m.GetElementsNodes("quad")
-    dfdu, dfdE = _response_grads(coords, conn, K, u.cpu().numpy())
-    rhs = dfdu.copy()
-    rhs[loss.dirichlet_indices] = 0.0
-    lam = s.LinearSolve(Kt, t64(rhs))  # solves Kt x = -rhs
-    _, gd = loss.ComputeResidualVJP(t64(K), u, lam)
-    chk = sa["test_sensitivites"]["checks"][0]
-    np.testing.assert_allclose(dfdE + gd.cpu().numpy(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
-    from tests.test_oracle_goldens import _response_shape_grad
-    gx = loss.ComputeResidualShapeVJP(t64(K), u, lam).cpu().numpy()
-    chk = sa["test_sensitivites"]["checks"][1]
-    dfdX = _response_shape_grad(coords, conn, K, u.cpu().numpy()).ravel()
-    np.testing.assert_allclose(dfdX + gx, chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+    adj = AdjointFiniteElementSolver("adj", resp, settings)
+    adj.Initialize()
+    u = s.Solve(K, np.zeros(2 * m.GetNumberOfNodes()))
+    lam = adj.Solve(K, u, torch.ones(2 * m.GetNumberOfNodes(), dtype=torch.float64, device="cuda"))
+    gd = resp.ComputeAdjointNodalControlDerivatives(K, u, lam)
+    gx = resp.ComputeAdjointNodalShapeDerivatives(K, u, lam)
+    for chk, actual in zip(sa["test_sensitivites"]["checks"][:2], (gd, gx)):
+        np.testing.assert_allclose(actual.cpu().numpy().ravel(), chk["expected"], rtol=chk["rtol"], atol=chk["atol"])
+
+
+def test_response_value_and_gradients_against_oracle():
+    """FiniteElementResponse on a distorted HEX8 mesh with a nonlinear formula: value, df/du, df/dk
+    and df/dX against central differences of the oracle's Gauss-point integral."""
+    from folax_amd import loss_functions as lf
+    from folax_amd.controls import FourierControl
+    from folax_amd.mesh import Mesh
+    from folax_amd.responses import FiniteElementResponse
+    from tests.cases import mesh_for
+    coords, conn, sets = mesh_for("hexahedron", 2, seed=51)
+    m = Mesh("m")
+    m.nodes_coordinates, m.elements_nodes, m.node_sets = coords, {"hexahedron": conn}, sets
+    m.node_ids = np.arange(coords.shape[0])
+    bc = {"Ux": {"bottom": 0.0}, "Uy": {"bottom": 0.0}, "Uz": {"bottom": 0.0, "top": 0.01}}
+    loss = lf.MechanicalLoss3DHexa("l", {"dirichlet_bc_dict": bc, "num_gp": 2,
+                                         "material_dict": {"young_modulus": 1.0, "poisson_ratio": 0.3}}, m)
+    loss.Initialize()
+    fc = FourierControl("K", {"x_freqs": np.array([1]), "y_freqs": np.array([1]), "z_freqs": np.array([1]),
+                              "beta": 1, "min": 0.1, "max": 1}, m)
+    formula = "jnp.sin(K) * U[2] + K * U[0] ** 2"
+    resp = FiniteElementResponse("r", formula, loss, fc)
+    resp.Initialize()
+    rng = np.random.default_rng(52)
+    k = rng.uniform(0.5, 1.5, coords.shape[0])
+    u = rng.uniform(-1, 1, 3 * coords.shape[0])
+
+    def f_ref(k, u, X):
+        N, w, DN, detJ = fe.geometry("hexahedron", X[conn], 2)
+        gd = np.einsum("ga,ea->eg", N, k[conn])
+        U = np.einsum("ga,eac->ceg", N, u.reshape(-1, 3)[conn])
+        return float(np.sum(w[None] * detJ * (np.sin(gd) * U[2] + gd * U[0] ** 2)))
+
+    assert abs(float(resp.ComputeValue(t64(k), t64(u))) - f_ref(k, u, coords)) <= 1e-12 * abs(f_ref(k, u, coords))
+    h = 1e-6
+    _, rhs = resp.ComputeAdjointJacobianMatrixAndRHSVector(t64(k), t64(u))
+    zero = np.zeros_like(u)
+    gk = resp.ComputeAdjointNodalControlDerivatives(t64(k), t64(u), t64(zero)).cpu().numpy()
+    gx = resp.ComputeAdjointNodalShapeDerivatives(t64(k), t64(u), t64(zero)).cpu().numpy().reshape(-1, 3)
+    free = np.setdiff1d(np.arange(u.size), loss.dirichlet_indices)
+    for v_u, v_k, v_x in [(rng.standard_normal(u.size), rng.standard_normal(k.size),
+                           rng.standard_normal(coords.shape)) for _ in range(3)]:
+        v_u[loss.dirichlet_indices] = 0.0
+        fd = (f_ref(k, u + h * v_u, coords) - f_ref(k, u - h * v_u, coords)) / (2 * h)
+        assert abs(-float(rhs.cpu().numpy()[free] @ v_u[free]) - fd) <= 1e-7 * max(1.0, abs(fd))
+        fd = (f_ref(k + h * v_k, u, coords) - f_ref(k - h * v_k, u, coords)) / (2 * h)
+        assert abs(float(gk @ v_k) - fd) <= 1e-7 * max(1.0, abs(fd))
+        fd = (f_ref(k, u, coords + h * v_x) - f_ref(k, u, coords - h * v_x)) / (2 * h)
+        assert abs(float(np.sum(gx * v_x)) - fd) <= 1e-7 * max(1.0, abs(fd))
 
 
 def _c1_inputs():
@@ -200,6 +237,56 @@ def test_batch_loss_value_and_autograd(kind, exponent):
             assert float(Pt.grad[b].abs().max()) == 0.0
         else:
             assert rel(Pt.grad[b].cpu().numpy(), scale * gd_ref) <= (1e-9 if kind == "thermal" else 1e-11)
+
+
+@pytest.mark.parametrize("kind", ["mechanical", "neohooke"])
+def test_parametric_boundary_learning(kind):
+    """fe_loss.py:91-103 with parametric_boundary_learning: the batch parameters are each sample's
+    Dirichlet values (known_dofs, in dirichlet_indices order) and the control is the constant
+    heterogeneity field (mechanical_saint_venant.py:59-66 pattern).  The loss equals the oracle's
+    at the overwritten dofs; dL/dknown is dE/du gathered at the Dirichlet dofs and dL/du is zero
+    there."""
+    from folax_amd import loss_functions as lf
+    from folax_amd.mesh import create_2D_square_mesh
+    m = create_2D_square_mesh(1.0, 6)
+    coords, conn = m.GetNodesCoordinates(), m.GetElementsNodes("quad")
+    bc = {"Ux": {"left": 0.0, "right": 0.05}, "Uy": {"left": 0.0, "right": 0.05}}
+    cls = lf.MechanicalLoss2DQuad if kind == "mechanical" else lf.NeoHookeMechanicalLoss2DQuad
+    md = {"young_modulus": 1.0, "poisson_ratio": 0.3, "heterogeneity_field_name": "E",
+          "heterogeneity_default_value": 1.5}
+    loss = cls("pbl", {"dirichlet_bc_dict": bc, "material_dict": md, "parametric_boundary_learning": True,
+                       "loss_function_exponent": 2.0}, m)
+    loss.Initialize()
+    idx = loss.dirichlet_indices
+    rng = np.random.default_rng(17)
+    B = 3
+    known = rng.uniform(-0.05, 0.05, size=(B, idx.size))
+    U = rng.uniform(-0.02, 0.02, size=(B, 2 * coords.shape[0]))
+    Kt, Ut = t64(known).requires_grad_(True), t64(U).requires_grad_(True)
+    mean, (lo, hi, _) = loss.ComputeBatchLoss(Kt, Ut)
+    mean.backward()
+    ph = fe.Physics(kind, "quad", E=1.0, nu=0.3)
+    field = np.full(coords.shape[0], 1.5)
+    es, gk, gu = [], [], []
+    for b in range(B):
+        full = U[b].copy()
+        full[idx] = known[b]
+        Eb = fe.total_energy(ph, coords, conn, field, full)
+        if kind == "neohooke":
+            _, dEdu, _ = fe.global_energy_grad(ph, coords, conn, field, full)
+        else:  # u^T stop_gradient(r): dE/du = r
+            _, dEdu = fe.assemble(ph, coords, conn, field, full, np.zeros(0, np.int64), want_K=False)
+        s = 2.0 * Eb / B
+        es.append(Eb ** 2)
+        gk.append(s * dEdu[idx])
+        g = s * dEdu
+        g[idx] = 0.0
+        gu.append(g)
+    assert abs(float(mean) - np.mean(es)) <= 1e-12 * np.mean(es)
+    assert abs(float(lo) - min(es)) <= 1e-12 * min(es) and abs(float(hi) - max(es)) <= 1e-12 * max(es)
+    assert rel(Kt.grad.cpu().numpy(), np.stack(gk)) <= 1e-11
+    assert rel(Ut.grad.cpu().numpy(), np.stack(gu)) <= 1e-11
+    np.testing.assert_array_equal(loss.GetFullDofVector(t64(known), t64(U)).cpu().numpy()[:, idx], known)
 
 
 @pytest.mark.parametrize("autotune", [False, True])

@@ -267,6 +267,29 @@ def test_hex48_fp32_matches_fp64(hex48):
     assert trel(a["pg"], b["pg"]) <= 1e-3
 
 
+def test_hex48_fp32_kernel_error_at_the_same_displacements(hex48):
+    """The FP32 kernels' own error at full size: the FP32 network's displacements U32 (rounded to
+    FP32 once) go through both the FP32 and the FP64 energy and energy-gradient kernels, so the
+    difference is the element arithmetic alone (the network's rounding does not enter).  Loss,
+    dL/dU and dL/dE within 1e-5 of FP64."""
+    loss = hex48["loss"]
+    a = hex48[torch.float32]
+    U32 = a["U"]
+    out = {}
+    for dt in (torch.float32, torch.float64):
+        Et = torch.as_tensor(hex48["E"], dtype=torch.float32, device="cuda").to(dt).requires_grad_(True)
+        Ut = U32.to(dt).detach().requires_grad_(True)
+        Lm, _ = loss.ComputeBatchLoss(Et, Ut)
+        Lm.backward()
+        out[dt] = (float(Lm), Ut.grad.double(), Et.grad.double())
+    L32, gU32, gE32 = out[torch.float32]
+    L64, gU64, gE64 = out[torch.float64]
+    assert abs(L32 - a["L"]) <= 1e-7 * abs(a["L"])  # the fixture's own FP32 forward pass
+    assert abs(L32 - L64) <= 1e-5 * abs(L64)
+    assert trel(gU32, gU64) <= 1e-5
+    assert trel(gE32, gE64) <= 1e-5
+
+
 def test_hex48_energy_is_homogeneous_in_the_field(hex48):
     """psi scales with the interpolated field (kappa, mu proportional to N.d): E . dL/dE = L."""
     for dt, tol in ((torch.float64, 1e-12), (torch.float32, 1e-5)):
