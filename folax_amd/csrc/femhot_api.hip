@@ -222,6 +222,34 @@ int femhot_energy_grad(femhot_ctx *h, int dtype, int batch, const void *field, l
   return run_elem(c, io);
 }
 
+int femhot_energy_grad_noalloc(femhot_ctx *h, int dtype, int batch, const void *field, long long fs, const void *u,
+                               long long us, double *energy, void *g_u, long long gus, void *g_field, long long gfs,
+                               unsigned flags, void *stream) {
+  FH_CHECK_ARG(h, FEMHOT_ERR_ARG, "null context");
+  Ctx *c = &h->c;
+  FH_CHECK_ARG(!(c->ph == THERMAL && c->pp.beta != 0.0), FEMHOT_ERR_UNSUPPORTED,
+               "femhot_energy_grad_noalloc: thermal conduction with beta != 0 has no potential");
+  CallIO io;
+  io.dtype = dtype;
+  io.batch = batch;
+  io.mode = MODE_EGRAD;
+  io.field = field;
+  io.fs = fs;
+  io.u = u;
+  io.us = us;
+  io.ou = g_u;
+  io.ous = gus;
+  io.od = g_field;
+  io.ods = gfs;
+  io.energy = energy;
+  io.flags = flags;
+  int rc = validate(c, io, "femhot_energy_grad_noalloc");
+  if (rc) return rc;
+  return run_chunk_fixed_scratch(c, dtype, MODE_EGRAD, batch, field, batch > 1 ? fs : 0, u, batch > 1 ? us : 0, g_u,
+                                 gus, g_field, gfs, energy, flags & ~(FEMHOT_SCATTER_COLOUR | FEMHOT_SCATTER_CHUNK),
+                                 (hipStream_t)stream);
+}
+
 int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *u, void *vals, void *R,
                        unsigned flags, void *stream) {
   FH_CHECK_ARG(h && field && u && vals, FEMHOT_ERR_ARG, "femhot_tangent_csr: null argument");

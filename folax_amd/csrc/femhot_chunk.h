@@ -20,6 +20,11 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
 
 int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int epc, int block, int grid_per_cu);
 
+// The chunk path on the library's fixed device scratch: allocates nothing, requires the call shape's
+// plan to be built, rejects calls whose partials exceed the fixed size.
+int run_chunk_fixed_scratch(Ctx *c, int dtype, int mode, int batch, const void *field, long long fs, const void *u,
+                            long long us, void *ou, long long ous, void *od, long long ods, double *energy,
+                            unsigned flags, hipStream_t s);
 
 void forget_tuning(const void *ctx);
 

@@ -515,4 +515,65 @@ void forget_tuning(const void *ctx) {
       ++it;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fixed scratch of the allocation-free entry point: the chunk path's shared-node partials and
+// per-chunk energies in a file-scope device array, one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host it is only a stand-in symbol):
+// its device address is resolved once per device with hipGetSymbolAddress and kept here.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t kFixedScratchBytes = 64ull << 20;  // 64 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];  // device address of `scratch` on each device, once resolved
+static std::mutex g_scratch_mu;
+
+int init_fixed_scratch() {
+  int dev = 0;
+  FH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("init_fixed_scratch: device index beyond the library's table");
+    return FEMHOT_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return FEMHOT_OK;
+  void *p = nullptr;
+  FH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[dev] = p;
+  return FEMHOT_OK;
+}
+
+static int fixed_scratch(void **out) {
+  int dev = 0;
+  FH_HIP(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev >= 0 && dev < kMaxDevices && g_scratch_ptr[dev]) {
+      *out = g_scratch_ptr[dev];
+      return FEMHOT_OK;
+    }
+  }
+  int rc = init_fixed_scratch();  // a context created on another device: resolve this one now
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  *out = g_scratch_ptr[dev];
+  return FEMHOT_OK;
+}
+
+int run_chunk_fixed_scratch(Ctx *c, int dtype, int mode, int batch, const void *field, long long fs, const void *u,
+                            long long us, void *ou, long long ous, void *od, long long ods, double *energy,
+                            unsigned flags, hipStream_t s) {
+  void *ws = nullptr;
+  int rc = fixed_scratch(&ws);
+  if (rc) return rc;
+  ChunkConfig cfg = default_config(c, dtype);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find({c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)});
+    if (it != g_tuned.end() && it->second.path == PATH_CHUNK) cfg = it->second;
+  }
+  return run_chunk_cfg(c, dtype, mode, batch, field, fs, u, us, nullptr, 0, ou, ous, od, ods, energy, flags, s, cfg,
+                       ws, kFixedScratchBytes);
+}
+
 }  // namespace femhot

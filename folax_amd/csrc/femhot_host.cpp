@@ -643,6 +643,10 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
     return fail(e, "hipMemcpy");
   if ((e = hipHostMalloc((void **)&c.h_pinned, 1024 * sizeof(double), hipHostMallocDefault)) != hipSuccess)
     return fail(e, "hipHostMalloc");
+  if (int rc = init_fixed_scratch()) {  // this device's address of the allocation-free path's scratch
+    femhot_ctx_destroy(h);
+    return rc;
+  }
   *out = h;
   return FEMHOT_OK;
 }

@@ -130,6 +130,9 @@ int ensure_chunks(Ctx *c, int epc, hipStream_t s, const ChunkPlan **out);
 // The cached plan of `epc` elements per chunk, or null when it is not built (never builds).
 const ChunkPlan *find_chunks(Ctx *c, int epc);
 void free_chunk_plans(Ctx *c);
+// Resolves, once per device with that device current, the address of the chunk path's fixed device
+// scratch (the allocation-free entry point's); called by femhot_ctx_create.
+int init_fixed_scratch();
 int ensure_workspace(Ctx *c, size_t bytes);
 
 // Records an event on `s` when timing is enabled (pairs: before/after the dominant kernel).

@@ -139,6 +139,24 @@ int femhot_residual_vjp(femhot_ctx *ctx, int dtype, int batch, const void *field
 int femhot_residual_shape_vjp(femhot_ctx *ctx, int dtype, const void *field, const void *u, const void *w,
                               void *g_x, unsigned flags, void *stream);
 
+/* femhot_energy_grad that allocates nothing: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as femhot_energy_grad on the LDS-staged chunk path (the
+ * configuration tuned or set for this call shape, else the default one; FEMHOT_SCATTER_* flags are
+ * ignored), but its scratch -- the shared-node partials and per-chunk energies -- lives in a fixed
+ * 64 MiB device array of the library, one per GPU, held until the process ends, instead of the
+ * context's growable workspace.  Requirements and limits:
+ *  - the chunk plan of the call shape must already be built (femhot_set_chunk_config, or one
+ *    femhot_energy_grad call with FEMHOT_SCATTER_CHUNK), else FEMHOT_ERR_ARG;
+ *  - a call whose scratch (batch x shared-node slots x (dofs per node [+1]) x element size + batch x
+ *    chunks x 8 bytes) exceeds 64 MiB returns FEMHOT_ERR_SHAPE;
+ *  - every call on one GPU shares that GPU's array: calls that overlap in time on one GPU (two
+ *    streams, two threads) race and must be serialised by the caller.  Nothing in the array needs
+ *    to start at zero: each call writes every slot it reads. */
+int femhot_energy_grad_noalloc(femhot_ctx *ctx, int dtype, int batch, const void *field, long long field_stride,
+                               const void *u, long long u_stride, double *energy, void *g_u, long long g_u_stride,
+                               void *g_field, long long g_field_stride, unsigned flags, void *stream);
+
 /* Integration data of every element at the context's Gauss rule (G points per element: QUAD4 g^2,
  * HEX8 g^3, TET4 1/4/8 for rule g = 1/2/3), DEVICE outputs in the arithmetic type, each optional:
  * N (G, nodes_per_elem) shape function values; wdetJ (n_elems, G) weight x signed detJ;
