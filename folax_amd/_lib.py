@@ -23,8 +23,8 @@ TRANSPOSE = 0x8
 ACCUMULATE = 0x10
 SCATTER_COLOUR = 0x20
 SCATTER_CHUNK = 0x40
-CHUNK_DMA = 0x10000
-PCG_CONTEXT_PATTERN = 0x100  # femhot_pcg mode flag: the matrix is on this context's pattern  # block | CHUNK_DMA: the pipelined LDS-DMA chunk kernel (FP32)
+CHUNK_DMA = 0x10000  # block | CHUNK_DMA: the pipelined LDS-DMA chunk kernel (FP32)
+PCG_CONTEXT_PATTERN = 0x100  # femhot_pcg mode flag: the matrix is on this context's pattern
 
 OK, ERR_ARG, ERR_SHAPE, ERR_INDEX, ERR_NONFINITE, ERR_HIP, ERR_NOT_CONVERGED, ERR_UNSUPPORTED = range(8)
 

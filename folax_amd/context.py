@@ -134,21 +134,27 @@ class FemhotContext:
         return self.n_elems if flags & L.ELEMENT_FIELD else self.n_nodes
 
     # ---------------- compute ----------------
-    def residual(self, field, u, want_R=True, want_energy=True, flags=0):
-        """R (B, n_dofs) and per-sample (per-segment) energies (float64)."""
+    def residual(self, field, u, want_R=True, want_energy=True, flags=0, out_R=None):
+        """R (B, n_dofs) and per-sample (per-segment) energies (float64).  out_R: a preallocated (B, n_dofs)
+        residual to write (graph-capture friendly)."""
         u = u if u.dim() == 2 else u.unsqueeze(0)
         field = field if field.dim() == 2 else field.unsqueeze(0)
         B = u.shape[0]
         dt, dev = u.dtype, u.device
         us = self._check_in(u, "u", self.n_dofs, B, dt, dev)
         fs = self._check_in(field, "field", self._field_size(flags), B, dt, dev)
-        R = torch.empty(B, self.n_dofs, dtype=dt, device=dev) if want_R else None
+        if out_R is not None:
+            self._check_in(out_R, "out_R", self.n_dofs, B, dt, dev)
+            if B > 1 and out_R.stride(0) != self.n_dofs:
+                raise ValueError("out_R must be contiguous")
+        R = out_R if out_R is not None else (torch.empty(B, self.n_dofs, dtype=dt, device=dev) if want_R else None)
         en = torch.empty(B * self.n_segments, dtype=torch.float64, device=dev) if want_energy else None
         L.check(self._lib.femhot_residual(self._h, _dtype_code(u), B, _ptr(field), fs, _ptr(u), us, _ptr(R),
                                           self.n_dofs, _ptr(en), flags, _stream()))
         return R, (en.view(B, self.n_segments) if en is not None else None)
 
-    def residual_vjp(self, field, u, w, want_gu=True, want_gd=True, flags=0):
+    def residual_vjp(self, field, u, w, want_gu=True, want_gd=True, flags=0, out=None):
+        """(dR/du)^T w and (dR/dfield)^T w; out: preallocated contiguous (g_u, g_field) to write."""
         u = u if u.dim() == 2 else u.unsqueeze(0)
         w = w if w.dim() == 2 else w.unsqueeze(0)
         field = field if field.dim() == 2 else field.unsqueeze(0)
@@ -158,8 +164,16 @@ class FemhotContext:
         ws = self._check_in(w, "w", self.n_dofs, B, dt, dev)
         fsz = self._field_size(flags)
         fs = self._check_in(field, "field", fsz, B, dt, dev)
-        gu = torch.empty(B, self.n_dofs, dtype=dt, device=dev) if want_gu else None
-        gd = torch.empty(B, fsz, dtype=dt, device=dev) if want_gd else None
+        if out is not None:
+            gu, gd = out
+            for t, name, n in ((gu, "g_u", self.n_dofs), (gd, "g_field", fsz)):
+                if t is not None:
+                    self._check_in(t, name, n, B, dt, dev)
+                    if B > 1 and t.stride(0) != n:
+                        raise ValueError(f"{name} must be contiguous")
+        else:
+            gu = torch.empty(B, self.n_dofs, dtype=dt, device=dev) if want_gu else None
+            gd = torch.empty(B, fsz, dtype=dt, device=dev) if want_gd else None
         L.check(self._lib.femhot_residual_vjp(self._h, _dtype_code(u), B, _ptr(field), fs, _ptr(u), us, _ptr(w), ws,
                                               _ptr(gu), self.n_dofs, _ptr(gd), fsz, flags, _stream()))
         return gu, gd

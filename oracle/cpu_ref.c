@@ -10,6 +10,10 @@
  * Pi_e = 1/2 u_e^T K_e u_e, its gradient r_e = K_e u_e scattered into R with OpenMP atomics (the
  * scatter pattern of fol/loss_functions/ffi_functions/kr_small_displacement_element.cc:81-93) and
  * dPi/dE_e written per element; per-segment potentials summed.
+ *
+ * quad4_thermal_resnorm_vjp: nonlinear thermal QUAD4 (fol/loss_functions/thermal.py:28-49), the
+ * batch residual-norm loss and its exact gradient with respect to the temperature and the nodal
+ * conductivity field (the same quantities as oracle/fe.py residual_norm_loss_grad).
  */
 #include <math.h>
 #include <stdint.h>
@@ -21,6 +25,133 @@ int cpuref_num_threads(void) {
 #else
   return 1;
 #endif
+}
+
+void cpuref_set_threads(int n) {
+#ifdef _OPENMP
+  extern void omp_set_num_threads(int);
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* QUAD4 2x2-Gauss geometry at Gauss point g: N (4), global gradients DN (4x2), weight x detJ
+ * (fol/geometries/quadrilateral_2d_4.py:17-66, geometry.py:87-97). */
+static double quad4_gp(const double x[4][2], int g, double N[4], double DN[4][2]) {
+  const double s = 0.57735026918962576451;
+  const int sx[4] = {-1, 1, 1, -1}, sy[4] = {-1, -1, 1, 1};
+  const double xi = sx[g] * s, eta = sy[g] * s;
+  double dN[4][2], J[2][2] = {{0, 0}, {0, 0}};
+  for (int a = 0; a < 4; ++a) {
+    N[a] = 0.25 * (1 + sx[a] * xi) * (1 + sy[a] * eta);
+    dN[a][0] = 0.25 * sx[a] * (1 + sy[a] * eta);
+    dN[a][1] = 0.25 * sy[a] * (1 + sx[a] * xi);
+    for (int i = 0; i < 2; ++i)
+      for (int k = 0; k < 2; ++k) J[i][k] += x[a][i] * dN[a][k];
+  }
+  const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double inv[2][2] = {{J[1][1] / det, -J[0][1] / det}, {-J[1][0] / det, J[0][0] / det}};
+  for (int a = 0; a < 4; ++a)
+    for (int j = 0; j < 2; ++j) DN[a][j] = dN[a][0] * inv[0][j] + dN[a][1] * inv[1][j];
+  return det; /* weight 1 */
+}
+
+static double powc(double x, double c) { return c == 2.0 ? x * x : (c == 1.0 ? x : pow(x, c)); }
+
+/* Nonlinear thermal QUAD4 (fol/loss_functions/thermal.py:28-49, k = (N.k0)(1 + beta (N.T)^c)):
+ * per sample b the Dirichlet-masked residual R_b = sum_e Se(k0_b, T_b) T_b, the loss
+ * L = 1/2 sum_b |R_b|^2, and its exact gradient: g_T = (dR/dT)^T R (zero at Dirichlet dofs) and
+ * g_k0 = (dR/dk0)^T R, through element residual VJPs scattered with OpenMP atomics.  T, k0, g_T,
+ * g_k0: (batch, n_nodes) row-major; R: (n_nodes) work array; dmask: 1 at Dirichlet nodes. */
+double quad4_thermal_resnorm_vjp(long long n_elems, const int *conn, const double *coords, long long n_nodes,
+                                 int batch, const double *k0, const double *T, double beta, double c,
+                                 const unsigned char *dmask, double *R, double *gT, double *gk0) {
+  double L = 0.0;
+  for (int b = 0; b < batch; ++b) {
+    const double *kb = k0 + (long long)b * n_nodes, *Tb = T + (long long)b * n_nodes;
+    double *gTb = gT + (long long)b * n_nodes, *gkb = gk0 + (long long)b * n_nodes;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n_nodes; ++i) R[i] = gTb[i] = gkb[i] = 0.0;
+#pragma omp parallel for schedule(static)
+    for (long long e = 0; e < n_elems; ++e) {
+      const int *nd = conn + 4 * e;
+      double x[4][2], Te[4], ke[4], r[4] = {0, 0, 0, 0};
+      for (int a = 0; a < 4; ++a) {
+        x[a][0] = coords[3 * (long long)nd[a]];
+        x[a][1] = coords[3 * (long long)nd[a] + 1];
+        Te[a] = Tb[nd[a]];
+        ke[a] = kb[nd[a]];
+      }
+      for (int g = 0; g < 4; ++g) {
+        double N[4], DN[4][2];
+        const double wd = quad4_gp(x, g, N, DN);
+        double dg = 0, Tg = 0, gx = 0, gy = 0;
+        for (int a = 0; a < 4; ++a) {
+          dg += N[a] * ke[a];
+          Tg += N[a] * Te[a];
+          gx += DN[a][0] * Te[a];
+          gy += DN[a][1] * Te[a];
+        }
+        const double k = dg * (1.0 + beta * powc(Tg, c));
+        for (int a = 0; a < 4; ++a) r[a] += wd * k * (DN[a][0] * gx + DN[a][1] * gy);
+      }
+      for (int a = 0; a < 4; ++a) {
+#pragma omp atomic
+        R[nd[a]] += r[a];
+      }
+    }
+    double lb = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : lb)
+    for (long long i = 0; i < n_nodes; ++i) {
+      if (dmask[i]) R[i] = 0.0;
+      lb += R[i] * R[i];
+    }
+    L += 0.5 * lb;
+#pragma omp parallel for schedule(static)
+    for (long long e = 0; e < n_elems; ++e) {
+      const int *nd = conn + 4 * e;
+      double x[4][2], Te[4], ke[4], we[4], gu[4] = {0, 0, 0, 0}, gd[4] = {0, 0, 0, 0};
+      for (int a = 0; a < 4; ++a) {
+        x[a][0] = coords[3 * (long long)nd[a]];
+        x[a][1] = coords[3 * (long long)nd[a] + 1];
+        Te[a] = Tb[nd[a]];
+        ke[a] = kb[nd[a]];
+        we[a] = R[nd[a]];
+      }
+      for (int g = 0; g < 4; ++g) {
+        double N[4], DN[4][2];
+        const double wd = quad4_gp(x, g, N, DN);
+        double dg = 0, Tg = 0, tx = 0, ty = 0, wx = 0, wy = 0;
+        for (int a = 0; a < 4; ++a) {
+          dg += N[a] * ke[a];
+          Tg += N[a] * Te[a];
+          tx += DN[a][0] * Te[a];
+          ty += DN[a][1] * Te[a];
+          wx += DN[a][0] * we[a];
+          wy += DN[a][1] * we[a];
+        }
+        const double f = 1.0 + beta * powc(Tg, c);
+        const double k = dg * f;
+        const double q = wd * (wx * tx + wy * ty);
+        const double dk = beta != 0.0 ? dg * beta * c * powc(Tg, c - 1.0) : 0.0;
+        for (int a = 0; a < 4; ++a) {
+          gu[a] += wd * k * (DN[a][0] * wx + DN[a][1] * wy) + q * dk * N[a];
+          gd[a] += q * f * N[a];
+        }
+      }
+      for (int a = 0; a < 4; ++a) {
+#pragma omp atomic
+        gTb[nd[a]] += gu[a];
+#pragma omp atomic
+        gkb[nd[a]] += gd[a];
+      }
+    }
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n_nodes; ++i)
+      if (dmask[i]) gTb[i] = 0.0;
+  }
+  return L;
 }
 
 void tet4_elastic_energy_grad(long long n_elems, const int *conn, const double *coords, const double *E,

@@ -19,8 +19,39 @@ def lib():
                                                d(np.float64, flags="C"), d(np.float64, flags="C")]
         L.tet4_elastic_energy_grad.restype = None
         L.cpuref_num_threads.restype = C.c_int
+        L.cpuref_set_threads.argtypes = [C.c_int]
+        L.cpuref_set_threads.restype = None
+        L.quad4_thermal_resnorm_vjp.argtypes = [C.c_longlong, d(np.int32, flags="C"), d(np.float64, flags="C"),
+                                                C.c_longlong, C.c_int, d(np.float64, flags="C"),
+                                                d(np.float64, flags="C"), C.c_double, C.c_double,
+                                                d(np.uint8, flags="C"), d(np.float64, flags="C"),
+                                                d(np.float64, flags="C"), d(np.float64, flags="C")]
+        L.quad4_thermal_resnorm_vjp.restype = C.c_double
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    lib().cpuref_set_threads(int(n))
+
+
+def quad4_thermal_resnorm_vjp(coords, conn, k0, T, dirichlet_nodes, beta=0.5, c=2.0, out=None):
+    """Nonlinear thermal QUAD4 residual-norm loss L = 1/2 sum_b |R_b|^2 over a batch (Dirichlet
+    entries of R masked) and its exact gradient (g_T, g_k0), FP64, OpenMP.  k0, T: (B, n_nodes)."""
+    k0 = np.ascontiguousarray(np.atleast_2d(k0), np.float64)
+    T = np.ascontiguousarray(np.atleast_2d(T), np.float64)
+    B, Nn = T.shape
+    mask = np.zeros(Nn, np.uint8)
+    mask[np.asarray(dirichlet_nodes, np.int64)] = 1
+    if out is None:
+        gT, gk = np.empty_like(T), np.empty_like(T)
+    else:
+        gT, gk = out
+    R = np.empty(Nn)
+    L = lib().quad4_thermal_resnorm_vjp(conn.shape[0], np.ascontiguousarray(conn, np.int32),
+                                        np.ascontiguousarray(coords, np.float64), Nn, B, k0, T, float(beta),
+                                        float(c), mask, R, gT, gk)
+    return L, gT, gk
 
 
 def d_coeffs(E, nu):

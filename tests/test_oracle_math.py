@@ -277,6 +277,29 @@ def test_c_restatement_matches_numpy_oracle():
     np.testing.assert_allclose(Pi, [Pe[seg == 0].sum(), Pe[seg == 1].sum()], rtol=1e-12)
 
 
+def test_c_thermal_restatement_matches_numpy_oracle():
+    """oracle/cpu_ref.c's nonlinear-thermal residual-norm loss and gradient (the CPU baseline of the
+    QUAD4 128^2 x 64 workload) equal the NumPy oracle on a distorted mesh, sample by sample, with
+    one thread and with several."""
+    from oracle import cpuref
+    case = make_case("thermal", "quad", n=6, seed=33, beta=0.5, c=2.0)
+    ph, coords, conn = case["ph"], case["coords"], case["conn"]
+    rng = np.random.default_rng(34)
+    B, Nn = 3, coords.shape[0]
+    K0 = rng.uniform(0.1, 1.0, size=(B, Nn))
+    T = rng.uniform(0.0, 1.0, size=(B, Nn))
+    T[:, case["didx"]] = case["dval"]
+    ref = [fe.residual_norm_loss_grad(ph, coords, conn, K0[b], T[b], case["didx"]) for b in range(B)]
+    default = cpuref.num_threads()
+    for threads in (1, 4):
+        cpuref.set_threads(threads)
+        L, gT, gk = cpuref.quad4_thermal_resnorm_vjp(coords, conn, K0, T, case["didx"], beta=0.5, c=2.0)
+        np.testing.assert_allclose(L, sum(r[0] for r in ref), rtol=1e-12)
+        np.testing.assert_allclose(gT, np.stack([r[1] for r in ref]), rtol=1e-11, atol=1e-14)
+        np.testing.assert_allclose(gk, np.stack([r[2] for r in ref]), rtol=1e-11, atol=1e-14)
+    cpuref.set_threads(default)
+
+
 def _wR_of_coords(case, coords):
     ph = case["ph"]
     dofs = fe.element_dofs(case["conn"], ph.ndpn)
