@@ -190,7 +190,7 @@ def traffic_if_same(prof, cfg):
 
 def pick_config(ctx, op, E, u, w, flags, args, prof, torch, L):
     """The chunk configuration of a workload: explicit flags, else the profiled one, else the autotuner's."""
-    if args.epc:
+    if args.epc or args.block in (L.PATH_ATOMIC, L.PATH_COLOUR):
         cfg = {"epc": args.epc, "block": args.block, "grid_per_cu": args.grid}
         src = "command line"
     elif prof and prof.get("config") and not args.autotune:
@@ -318,7 +318,7 @@ def c4_profile(args):
     Et = torch.as_tensor(E, dtype=torch.float32, device=dev)
     U = torch.as_tensor(np.random.default_rng(4300).uniform(-1e-3, 1e-3, size=(B, 3 * m.GetNumberOfNodes())),
                         dtype=torch.float32, device=dev)
-    if args.epc:
+    if args.epc or args.block in (L.PATH_ATOMIC, L.PATH_COLOUR):
         ctx.set_chunk_config(L.OP_ENERGY_GRAD, torch.float32, B, args.epc, args.block, args.grid)
         log(f"C4 config: {args.epc} elements/chunk, block {args.block}, grid {args.grid}")
     else:
@@ -349,7 +349,8 @@ def main():
     ap.add_argument("--profile", choices=["c5", "c3", "c4"], default=None,
                     help="short run of one workload's kernels for rocprofv3 (c5: + a 256 MiB copy for calibration)")
     ap.add_argument("--epc", type=int, default=0, help="fixed elements per chunk")
-    ap.add_argument("--block", type=int, default=256, help="with --epc: threads (| FEMHOT_CHUNK_ACC / _DMA)")
+    ap.add_argument("--block", type=int, default=256, help="with --epc: threads (| FEMHOT_CHUNK_DMA); with --epc 0: 1 = atomic, 2 = colour-ordered "
+                         "element-parallel path")
     ap.add_argument("--grid", type=int, default=0, help="with --epc: workgroups per CU (0: one per chunk)")
     ap.add_argument("--solver-profile", action="store_true",
                     help="short run for rocprofv3 of the linear-solver kernels instead: HEX8 64^3 two-phase "
