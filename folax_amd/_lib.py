@@ -23,6 +23,7 @@ TRANSPOSE = 0x8
 ACCUMULATE = 0x10
 SCATTER_COLOUR = 0x20
 SCATTER_CHUNK = 0x40
+SCATTER_ROWS = 0x80  # tangent: row-owner assembly (each CSR value summed in a fixed order, written once)
 CHUNK_DMA = 0x10000  # block | CHUNK_DMA: the pipelined LDS-DMA chunk kernel (FP32)
 PCG_CONTEXT_PATTERN = 0x100  # femhot_pcg mode flag: the matrix is on this context's pattern
 
@@ -41,6 +42,7 @@ EXPORTED = [
 
 OP_RESIDUAL, OP_VJP, OP_ENERGY_GRAD, OP_TANGENT = 0, 1, 2, 3
 PATH_ATOMIC, PATH_COLOUR = 1, 2  # autotune's block when an element-parallel path won (epc = 0)
+PATH_ROWS = 3  # ... and for the tangent, when the row-owner assembly won
 
 
 def gauss_points_per_element(elem_type, num_gp):

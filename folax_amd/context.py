@@ -261,8 +261,9 @@ class FemhotContext:
         return epc.value, blk.value, grid.value, ms.value
 
     def autotune_tangent(self, field, u, flags=0):
-        """Times atomic against colour-ordered tangent assembly on these inputs and keeps the faster
-        for tangent_csr calls with SCATTER_CHUNK.  Returns (L.PATH_ATOMIC or L.PATH_COLOUR, ms)."""
+        """Times atomic, colour-ordered and (thermal, linear elastic) row-owner tangent assembly on these
+        inputs and keeps the fastest for tangent_csr calls with SCATTER_CHUNK.  Returns (L.PATH_ATOMIC,
+        L.PATH_COLOUR or L.PATH_ROWS, ms)."""
         field = field.reshape(-1).contiguous()
         u = u.reshape(-1).contiguous()
         vals = torch.empty(max(self.nnz(), 1), dtype=u.dtype, device=u.device)
@@ -272,14 +273,23 @@ class FemhotContext:
                                           C.byref(grid), C.byref(ms), _stream()))
         return blk.value, ms.value
 
-    def tangent_csr(self, field, u, want_R=True, flags=0):
-        """CSR values (nnz) on the pattern of `pattern()`, BC-masked, and the masked residual."""
+    def tangent_csr(self, field, u, want_R=True, flags=0, out_vals=None):
+        """CSR values (nnz) on the pattern of `pattern()`, BC-masked, and the masked residual.  out_vals:
+        a contiguous (nnz) tensor to write (or, with ACCUMULATE, to add) the values into."""
         field = field.reshape(-1)
         u = u.reshape(-1)
         dt, dev = u.dtype, u.device
         self._check_in(u.view(1, -1), "u", self.n_dofs, 1, dt, dev)
         self._check_in(field.view(1, -1), "field", self._field_size(flags), 1, dt, dev)
-        vals = torch.empty(max(self.nnz(), 1), dtype=dt, device=dev)
+        if out_vals is None:
+            if flags & L.ACCUMULATE:
+                raise ValueError("tangent_csr: ACCUMULATE adds into out_vals, which is not given")
+            vals = torch.empty(max(self.nnz(), 1), dtype=dt, device=dev)
+        else:
+            if not out_vals.is_contiguous():
+                raise ValueError("tangent_csr: out_vals must be contiguous")
+            self._check_in(out_vals.view(1, -1), "out_vals", self.nnz(), 1, dt, dev)
+            vals = out_vals
         R = torch.empty(self.n_dofs, dtype=dt, device=dev) if want_R else None
         L.check(self._lib.femhot_tangent_csr(self._h, _dtype_code(u), _ptr(field), _ptr(u), _ptr(vals), _ptr(R),
                                              flags, _stream()))

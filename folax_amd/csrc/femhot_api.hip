@@ -260,11 +260,17 @@ int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *
   hipStream_t s = (hipStream_t)stream;
   const bool acc = (flags & FEMHOT_ACCUMULATE) != 0;
   // FEMHOT_SCATTER_CHUNK leaves the assembly order to the library: the path femhot_autotune
-  // measured fastest for the tangent (colour-ordered untuned)
-  if ((flags & FEMHOT_SCATTER_CHUNK) && !(flags & FEMHOT_SCATTER_COLOUR) &&
-      tuned_path(c, dtype, MODE_TANGENT, 1, flags) == PATH_COLOUR)
-    flags |= FEMHOT_SCATTER_COLOUR;
-  if (!acc) {
+  // measured fastest for the tangent (untuned: row-owner, colour-ordered for Neo-Hookean)
+  if ((flags & FEMHOT_SCATTER_CHUNK) && !(flags & (FEMHOT_SCATTER_COLOUR | FEMHOT_SCATTER_ROWS))) {
+    const int path = tuned_path(c, dtype, MODE_TANGENT, 1, flags);
+    if (path == PATH_COLOUR) flags |= FEMHOT_SCATTER_COLOUR;
+    if (path == PATH_ROWS) flags |= FEMHOT_SCATTER_ROWS;
+  }
+  // row-owner assembly covers thermal and linear elasticity (and grids of up to 2^26 nodes of 64
+  // lanes); otherwise the call takes colour-ordered adds
+  const bool rows = (flags & FEMHOT_SCATTER_ROWS) && c->ph != NEOHOOKE && c->Nn <= (1LL << 26);
+  if ((flags & FEMHOT_SCATTER_ROWS) && !rows) flags |= FEMHOT_SCATTER_COLOUR;
+  if (!acc && !rows) {
     rc = dtype == FEMHOT_F32 ? fill<float>((float *)vals, c->nnz, c->nnz, 1, 0.f, s)
                              : fill<double>((double *)vals, c->nnz, c->nnz, 1, 0.0, s);
     if (rc) return rc;
@@ -283,12 +289,23 @@ int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *
   A.row_ptr = c->d_row_ptr;
   A.slot = c->d_slot;
   A.dmask = c->d_dmask;
+  A.ne_off = c->d_ne_off;
+  A.ne_list = c->d_ne_list;
+  A.n_nodes = c->Nn;
   A.flags = flags;
   A.pp = c->pp;
   const int n = c->n;
-  // colour-ordered plain adds need distinct nodes within an element (lanes of one element would
-  // otherwise update the same entries); meshes with collapsed elements assemble with atomics
-  if ((flags & FEMHOT_SCATTER_COLOUR) && !c->repeated_nodes) {
+  L.rows = false;
+  if (rows) {  // one 64-lane workgroup per node
+    L.rows = true;
+    L.plain = true;
+    A.list = nullptr;
+    A.count = c->Ne;
+    L.grid = dim3((unsigned)c->Nn);
+    if (c->Nn && (rc = launch_tangent_any(c->et, L))) return rc;
+  } else if ((flags & FEMHOT_SCATTER_COLOUR) && !c->repeated_nodes) {
+    // colour-ordered plain adds need distinct nodes within an element (lanes of one element would
+    // otherwise update the same entries); meshes with collapsed elements assemble with atomics
     if ((rc = ensure_colouring(c))) return rc;
     L.plain = true;
     for (int k = 0; k < c->n_colours; ++k) {
@@ -345,25 +362,32 @@ int femhot_autotune(femhot_ctx *h, int op, int dtype, int batch, const void *fie
     *ms_out = t / reps;
     return FEMHOT_OK;
   };
-  if (op == FEMHOT_OP_TANGENT) {  // atomic against colour-ordered assembly; out_u holds the values
+  if (op == FEMHOT_OP_TANGENT) {  // atomic, colour-ordered and row-owner assembly; out_u holds the values
     FH_CHECK_ARG(field && u && out_u, FEMHOT_ERR_ARG, "femhot_autotune: tangent needs field, u and values");
     const unsigned tf = flags & (FEMHOT_ELEMENT_FIELD | FEMHOT_TRANSPOSE);
     FH_HIP(hipEventCreate(&e0));
     FH_HIP(hipEventCreate(&e1));
-    float t_atomic = 0.f, t_colour = 0.f;
+    const bool rows_ok = c->ph != NEOHOOKE && c->Nn <= (1LL << 26);
+    float t_atomic = 0.f, t_colour = 0.f, t_rows = 0.f;
     int rc = time_it([&] { return femhot_tangent_csr(h, dtype, field, u, out_u, nullptr, tf, stream); }, &t_atomic);
     if (!rc)
       rc = time_it([&] { return femhot_tangent_csr(h, dtype, field, u, out_u, nullptr, tf | FEMHOT_SCATTER_COLOUR,
                                                    stream); }, &t_colour);
+    if (!rc && rows_ok)
+      rc = time_it([&] { return femhot_tangent_csr(h, dtype, field, u, out_u, nullptr, tf | FEMHOT_SCATTER_ROWS,
+                                                   stream); }, &t_rows);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    const int path = t_colour <= t_atomic ? PATH_COLOUR : PATH_ATOMIC;
+    int path = t_colour <= t_atomic ? PATH_COLOUR : PATH_ATOMIC;
+    float best = std::min(t_colour, t_atomic);
+    if (rows_ok && t_rows <= best) path = PATH_ROWS, best = t_rows;
     record_path(c, dtype, MODE_TANGENT, 1, flags, path);
     if (epc) *epc = 0;
-    if (block) *block = path == PATH_COLOUR ? FEMHOT_PATH_COLOUR : FEMHOT_PATH_ATOMIC;
+    if (block)
+      *block = path == PATH_ROWS ? FEMHOT_PATH_ROWS : path == PATH_COLOUR ? FEMHOT_PATH_COLOUR : FEMHOT_PATH_ATOMIC;
     if (grid_per_cu) *grid_per_cu = 0;
-    if (ms) *ms = std::min(t_colour, t_atomic);
+    if (ms) *ms = best;
     return FEMHOT_OK;
   }
   CallIO io;

@@ -31,14 +31,14 @@ void forget_tuning(const void *ctx);
 // The scatter a call takes when FEMHOT_SCATTER_CHUNK leaves the choice to the library: the LDS-staged
 // chunk kernel, the element-parallel kernel with hardware atomics, or the element-parallel kernel one
 // colour class at a time with plain read-modify-writes.
-enum ScatterPath { PATH_CHUNK = 0, PATH_ATOMIC = 1, PATH_COLOUR = 2 };
+enum ScatterPath { PATH_CHUNK = 0, PATH_ATOMIC = 1, PATH_COLOUR = 2, PATH_ROWS = 3 /* tangent only */ };
 // mode of the tangent assembly in the tuning table (next to MODE_RES / MODE_VJP / MODE_EGRAD)
 constexpr int MODE_TANGENT = 3;
 
 // The path femhot_autotune measured fastest for this call shape; untuned: the chunk kernel, except
 // FP64 Neo-Hookean (its element routine needs most of the register file, so only small chunks with
 // small workgroups keep the chunk kernel ahead) which takes the atomic path, and the tangent, which
-// takes the colour-ordered path.
+// takes the row-owner assembly (colour-ordered adds for Neo-Hookean).
 int tuned_path(const Ctx *c, int dtype, int mode, int batch, unsigned flags);
 
 // Records the path that won the autotuner for this call shape.

@@ -458,12 +458,14 @@ int autotune_chunk(Ctx *c, int dtype, int mode, int batch, const void *field, lo
 
 int set_chunk_config(Ctx *c, int mode, int dtype, int batch, unsigned flags, int epc, int block, int grid_per_cu) {
   if (epc == 0 || mode == MODE_TANGENT) {  // an element-parallel path, as femhot_autotune reports it
-    if (block != FEMHOT_PATH_ATOMIC && block != FEMHOT_PATH_COLOUR) {
-      set_error("femhot_set_chunk_config: epc 0 (or the tangent) takes block FEMHOT_PATH_ATOMIC or FEMHOT_PATH_COLOUR");
+    if (block != FEMHOT_PATH_ATOMIC && block != FEMHOT_PATH_COLOUR &&
+        !(mode == MODE_TANGENT && block == FEMHOT_PATH_ROWS)) {
+      set_error("femhot_set_chunk_config: epc 0 (or the tangent) takes block FEMHOT_PATH_ATOMIC or FEMHOT_PATH_COLOUR "
+                "(the tangent also FEMHOT_PATH_ROWS)");
       return FEMHOT_ERR_ARG;
     }
     record_path(c, dtype, mode, mode == MODE_TANGENT ? 1 : batch, flags,
-                block == FEMHOT_PATH_COLOUR ? PATH_COLOUR : PATH_ATOMIC);
+                block == FEMHOT_PATH_ROWS ? PATH_ROWS : block == FEMHOT_PATH_COLOUR ? PATH_COLOUR : PATH_ATOMIC);
     return FEMHOT_OK;
   }
   const bool dma = (block & (int)FEMHOT_CHUNK_DMA) != 0;
@@ -492,7 +494,7 @@ int tuned_path(const Ctx *c, int dtype, int mode, int batch, unsigned flags) {
     auto it = g_tuned.find({c, mode, dtype, batch, (int)(flags & FEMHOT_ELEMENT_FIELD)});
     if (it != g_tuned.end()) return it->second.path;
   }
-  if (mode == MODE_TANGENT) return PATH_COLOUR;
+  if (mode == MODE_TANGENT) return c->ph == NEOHOOKE ? PATH_COLOUR : PATH_ROWS;
   return (dtype == FEMHOT_F64 && c->ph == NEOHOOKE) ? PATH_ATOMIC : PATH_CHUNK;
 }
 

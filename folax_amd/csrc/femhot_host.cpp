@@ -29,15 +29,26 @@ int ensure_pattern(Ctx *c) {
   if (c->have_pattern) return FEMHOT_OK;
   const int n = c->n, P = c->ndpn;
   const long long Nn = c->Nn, Ne = c->Ne;
+  if (Ne * n > 2147483647LL) {
+    set_error("ensure_pattern: more than 2^31 - 1 element corners");
+    return FEMHOT_ERR_SHAPE;
+  }
   std::vector<long long> cnt(Nn + 1, 0);
   for (long long e = 0; e < Ne; ++e)
     for (int a = 0; a < n; ++a) cnt[c->h_conn[e * n + a] + 1]++;
   for (long long i = 0; i < Nn; ++i) cnt[i + 1] += cnt[i];
   std::vector<long long> n2e(cnt[Nn]);
+  // node -> (element, local index) incidence of the row-owner tangent assembly, element order
+  std::vector<int> ne_list(cnt[Nn]), ne_off(Nn + 1);
   {
     std::vector<long long> pos(cnt.begin(), cnt.end() - 1);
     for (long long e = 0; e < Ne; ++e)
-      for (int a = 0; a < n; ++a) n2e[pos[c->h_conn[e * n + a]]++] = e;
+      for (int a = 0; a < n; ++a) {
+        const long long k = pos[c->h_conn[e * n + a]]++;
+        n2e[k] = e;
+        ne_list[k] = (int)(e * n + a);
+      }
+    for (long long i = 0; i <= Nn; ++i) ne_off[i] = (int)cnt[i];
   }
   std::vector<long long> adj_off(Nn + 1, 0);
   std::vector<int> adj;
@@ -99,6 +110,11 @@ int ensure_pattern(Ctx *c) {
   FH_HIP(hipMemcpy(c->d_row_ptr, c->h_row_ptr.data(), sizeof(long long) * (c->Ndof + 1), hipMemcpyHostToDevice));
   if (base) FH_HIP(hipMemcpy(c->d_col_idx, c->h_col_idx.data(), sizeof(int) * base, hipMemcpyHostToDevice));
   FH_HIP(hipMemcpy(c->d_slot, slot.data(), slot.size(), hipMemcpyHostToDevice));
+  FH_HIP(hipMalloc(&c->d_ne_off, sizeof(int) * (Nn + 1)));
+  FH_HIP(hipMalloc(&c->d_ne_list, sizeof(int) * std::max<size_t>(1, ne_list.size())));
+  FH_HIP(hipMemcpy(c->d_ne_off, ne_off.data(), sizeof(int) * (Nn + 1), hipMemcpyHostToDevice));
+  if (!ne_list.empty())
+    FH_HIP(hipMemcpy(c->d_ne_list, ne_list.data(), sizeof(int) * ne_list.size(), hipMemcpyHostToDevice));
   c->have_pattern = true;
   return FEMHOT_OK;
 }
@@ -663,6 +679,8 @@ void femhot_ctx_destroy(femhot_ctx *h) {
   (void)hipFree(c.d_row_ptr);
   (void)hipFree(c.d_col_idx);
   (void)hipFree(c.d_slot);
+  (void)hipFree(c.d_ne_off);
+  (void)hipFree(c.d_ne_list);
   (void)hipFree(c.d_colour_elems);
   free_chunk_plans(&c);
   (void)hipFree(c.chunk_ws.ptr);

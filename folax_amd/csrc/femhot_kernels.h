@@ -226,6 +226,8 @@ struct TArgs {
   const long long *row_ptr;
   const uint8_t *slot;
   const uint8_t *dmask;
+  const int *ne_off, *ne_list;  // row-owner assembly: node -> (element * n + local index)
+  long long n_nodes;
   unsigned flags;
   PhysParams pp;
 };

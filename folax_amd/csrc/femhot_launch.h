@@ -3,6 +3,7 @@
 #include <type_traits>
 
 #include "femhot_kernels.h"
+#include "femhot_rows.h"
 
 namespace femhot {
 
@@ -17,6 +18,7 @@ struct ElemLaunch {
 struct TangentLaunch {
   int dtype, ph, ngp;
   bool plain;
+  bool rows;  // row-owner assembly (k_tangent_rows, one 64-lane workgroup per node); grid = nodes
   dim3 grid;
   hipStream_t stream;
   TArgs args;
@@ -72,6 +74,16 @@ template <int ET> int launch_tangent_impl(const TangentLaunch &L) {
   return dispatch_r_ph_ngp(L.dtype, L.ph, L.ngp, [&](auto rp, auto phc, auto ngpc) -> int {
     using R = std::remove_pointer_t<decltype(rp)>;
     constexpr int PH = decltype(phc)::value, NGP = decltype(ngpc)::value;
+    if (L.rows) {
+      if constexpr (PH == NEOHOOKE) {
+        set_error("femhot_tangent_csr: the row-owner assembly covers thermal and linear-elastic physics");
+        return FEMHOT_ERR_UNSUPPORTED;
+      } else {
+        hipLaunchKernelGGL((k_tangent_rows<R, ET, PH, NGP>), L.grid, dim3(64), 0, L.stream, L.args);
+        FH_HIP(hipGetLastError());
+        return FEMHOT_OK;
+      }
+    }
     void (*k)(TArgs) = L.plain ? k_tangent<R, ET, PH, NGP, true> : k_tangent<R, ET, PH, NGP, false>;
     hipLaunchKernelGGL(k, L.grid, dim3(256), 0, L.stream, L.args);
     FH_HIP(hipGetLastError());

@@ -546,9 +546,10 @@ extern "C" int femhot_newton(femhot_ctx *h, int mode, const double *field, doubl
                "femhot_newton: mode must be FEMHOT_F64 or FEMHOT_F32");
   FH_CHECK_ARG(load_steps > 0 && max_iter > 0 && lin_tol > 0 && lin_maxiter > 0, FEMHOT_ERR_ARG,
                "femhot_newton: load_steps, max_iter, lin_tol and lin_maxiter must be positive");
-  FH_CHECK_ARG((flags & ~(FEMHOT_ELEMENT_FIELD | FEMHOT_SCATTER_COLOUR | FEMHOT_SCATTER_CHUNK)) == 0,
+  FH_CHECK_ARG((flags & ~(FEMHOT_ELEMENT_FIELD | FEMHOT_SCATTER_COLOUR | FEMHOT_SCATTER_CHUNK | FEMHOT_SCATTER_ROWS)) ==
+                   0,
                FEMHOT_ERR_ARG,
-               "femhot_newton: flags may hold FEMHOT_ELEMENT_FIELD, FEMHOT_SCATTER_COLOUR and FEMHOT_SCATTER_CHUNK only");
+               "femhot_newton: flags may hold FEMHOT_ELEMENT_FIELD and FEMHOT_SCATTER_{COLOUR,CHUNK,ROWS} only");
   Ctx *c = &h->c;
   int rc = ensure_pattern(c);
   if (rc) return rc;

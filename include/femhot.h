@@ -61,6 +61,8 @@ extern "C" {
 #define FEMHOT_ACCUMULATE 0x10u     /* add into outputs instead of overwriting them */
 #define FEMHOT_SCATTER_COLOUR 0x20u /* colour-ordered, non-atomic, deterministic scatter */
 #define FEMHOT_SCATTER_CHUNK 0x40u  /* LDS-staged chunk scatter (persistent launch) */
+#define FEMHOT_SCATTER_ROWS 0x80u   /* tangent: row-owner assembly, each CSR value summed in a fixed
+                                       order and written once (thermal, linear elastic) */
 
 /* status codes */
 #define FEMHOT_OK 0
@@ -120,7 +122,13 @@ int femhot_residual(femhot_ctx *ctx, int dtype, int batch, const void *field, lo
                     unsigned flags, void *stream);
 
 /* Tangent values on the CSR pattern (+ optional residual R), BC-masked exactly like
- * fe_loss.py:191-207; FEMHOT_TRANSPOSE transposes each element matrix first. */
+ * fe_loss.py:191-207; FEMHOT_TRANSPOSE transposes each element matrix first.  Assembly: hardware
+ * atomics into zero-filled values (no scatter flag), colour-ordered plain adds
+ * (FEMHOT_SCATTER_COLOUR), or row-owner assembly (FEMHOT_SCATTER_ROWS: one workgroup per node sums
+ * the node's rows from its incident elements in element order and writes each value once; no fill,
+ * deterministic; thermal and linear elastic, Neo-Hookean falls back to colour-ordered adds);
+ * FEMHOT_SCATTER_CHUNK takes the one femhot_autotune measured fastest (untuned: row-owner, for
+ * Neo-Hookean colour-ordered). */
 int femhot_tangent_csr(femhot_ctx *ctx, int dtype, const void *field, const void *u, void *vals, void *R,
                        unsigned flags, void *stream);
 
@@ -196,7 +204,7 @@ int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *c
  * (the two norms) besides the linear solver's own convergence checks.
  * field: FP64 DEVICE, (n_nodes) or (n_elems) with FEMHOT_ELEMENT_FIELD.  u: FP64 DEVICE (n_dofs),
  * initial guess in, solution out.  flags: FEMHOT_ELEMENT_FIELD | FEMHOT_SCATTER_COLOUR |
- * FEMHOT_SCATTER_CHUNK only (the last: the tangent path femhot_autotune measured, else colour-ordered).
+ * FEMHOT_SCATTER_ROWS | FEMHOT_SCATTER_CHUNK only: the tangent assembly, as for femhot_tangent_csr.
  * history: HOST (load_steps x max_iter x 2) doubles or NULL: (||R||, ||du||) per iteration, NaN
  * after a load step's last iteration.  steps: HOST (load_steps) ints or NULL: iterations taken per
  * load step.  Synchronises the stream. */
@@ -213,9 +221,9 @@ int femhot_newton(femhot_ctx *ctx, int mode, const double *field, double *u, uns
  * element-parallel path won, *block then FEMHOT_PATH_ATOMIC or FEMHOT_PATH_COLOUR.  Untuned,
  * FEMHOT_SCATTER_CHUNK calls take the chunk path except in FP64 Neo-Hookean, which takes the
  * atomic path.  Finally runs the operation once more so the outputs are valid.
- * op FEMHOT_OP_TANGENT times femhot_tangent_csr with atomics against colour-ordered assembly (out_u
- * = the CSR values, no residual) and keeps the faster for femhot_tangent_csr calls with
- * FEMHOT_SCATTER_CHUNK (untuned: colour-ordered).  Synchronises the stream (not for capture).
+ * op FEMHOT_OP_TANGENT times femhot_tangent_csr with atomics, colour-ordered and (thermal, linear
+ * elastic) row-owner assembly (out_u = the CSR values, no residual) and keeps the fastest for
+ * femhot_tangent_csr calls with FEMHOT_SCATTER_CHUNK.  Synchronises the stream (not for capture).
  * op: FEMHOT_OP_RESIDUAL / FEMHOT_OP_VJP / FEMHOT_OP_ENERGY_GRAD / FEMHOT_OP_TANGENT.  No
  * counterpart in the reference (XLA fuses the element vmap itself); it replaces fe_loss.py:105-114's
  * fixed batch size.  In FP32 the candidates include the software-pipelined chunk kernel, whose
@@ -224,10 +232,12 @@ int femhot_newton(femhot_ctx *ctx, int mode, const double *field, double *u, uns
 #define FEMHOT_OP_RESIDUAL 0
 #define FEMHOT_OP_VJP 1
 #define FEMHOT_OP_ENERGY_GRAD 2
-#define FEMHOT_OP_TANGENT 3   /* femhot_tangent_csr: atomic against colour-ordered assembly; out_u = values */
-/* *block when an element-parallel path won (*epc = 0): hardware atomics, or colour-ordered plain adds */
+#define FEMHOT_OP_TANGENT 3   /* femhot_tangent_csr: atomic, colour-ordered and row-owner assembly; out_u = values */
+/* *block when an element-parallel path won (*epc = 0): hardware atomics, or colour-ordered plain adds;
+ * for the tangent also the row-owner assembly */
 #define FEMHOT_PATH_ATOMIC 1
 #define FEMHOT_PATH_COLOUR 2
+#define FEMHOT_PATH_ROWS 3
 int femhot_autotune(femhot_ctx *ctx, int op, int dtype, int batch, const void *field, long long field_stride,
                     const void *u, long long u_stride, const void *w, long long w_stride, void *out_u,
                     long long out_u_stride, void *out_field, long long out_field_stride, double *energy,
@@ -239,7 +249,8 @@ int femhot_autotune(femhot_ctx *ctx, int op, int dtype, int batch, const void *f
  * FEMHOT_CHUNK_DMA for the pipelined LDS-DMA kernel (FP32 only).  grid_per_cu: 0 = one workgroup
  * per work item, k > 0 = k persistent workgroups per CU, k < 0 = -k persistent workgroups in all.
  * epc = 0 (and always for op FEMHOT_OP_TANGENT) records an element-parallel path instead, block =
- * FEMHOT_PATH_ATOMIC or FEMHOT_PATH_COLOUR, as femhot_autotune reports it.
+ * FEMHOT_PATH_ATOMIC or FEMHOT_PATH_COLOUR (for the tangent also FEMHOT_PATH_ROWS), as
+ * femhot_autotune reports it.
  * No counterpart in the reference. */
 int femhot_set_chunk_config(femhot_ctx *ctx, int op, int dtype, int batch, unsigned flags, int epc, int block,
                             int grid_per_cu);

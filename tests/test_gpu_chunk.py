@@ -163,11 +163,11 @@ def test_scatter_path_recorded_and_taken():
     Pi, R, gd = fe.global_energy_grad(case["ph"], case["coords"], case["conn"], case["field"], case["u"])
     assert rel(gu_k.cpu().numpy(), R) <= 1e-12 and rel(gd_k.cpu().numpy(), gd) <= 1e-12
     path, ms = ctx.autotune_tangent(f[0], u[0])
-    assert path in (L.PATH_ATOMIC, L.PATH_COLOUR) and ms > 0
+    assert path in (L.PATH_ATOMIC, L.PATH_COLOUR, L.PATH_ROWS) and ms > 0
     vk, Rk = ctx.tangent_csr(f, u, flags=CHUNK)
-    vc, Rc = ctx.tangent_csr(f, u, flags=COLOUR if path == L.PATH_COLOUR else 0)
+    vc, Rc = ctx.tangent_csr(f, u, flags={L.PATH_COLOUR: COLOUR, L.PATH_ROWS: L.SCATTER_ROWS}.get(path, 0))
     torch.cuda.synchronize()
-    if path == L.PATH_COLOUR:
+    if path in (L.PATH_COLOUR, L.PATH_ROWS):  # both orders are fixed: same bits
         assert torch.equal(vk, vc)
     assert rel(vk.cpu().numpy(), vc.cpu().numpy()) <= 1e-12
     # the whole autotuner reports a consistent winner and leaves valid outputs

@@ -126,6 +126,24 @@ def test_hex64_tangent_null_space_symmetry_and_residual(hex64):
     assert trel(hex64["ctx"].spmv(rp, ci, vals, u), Ku) <= 1e-14
 
 
+def test_hex64_row_owner_assembly(hex64):
+    """Row-owner tangent assembly at full size: every value written once from a fixed summation order,
+    so two calls give the same bits; equal to the atomic assembly within FP64 rounding; with
+    ACCUMULATE it adds onto what the values hold."""
+    from folax_amd import _lib as L
+    ctx, E, u = hex64["ctx"], hex64["E"], hex64["u"]
+    v1, R1 = ctx.tangent_csr(E, u, flags=L.SCATTER_ROWS)
+    v2, _ = ctx.tangent_csr(E, u, want_R=False, flags=L.SCATTER_ROWS)
+    torch.cuda.synchronize()
+    assert torch.equal(v1, v2)
+    assert trel(v1, hex64["vals"]) <= 1e-13
+    assert trel(R1, hex64["R"]) <= 1e-13
+    acc = v1.clone()
+    ctx.tangent_csr(E, u, want_R=False, flags=L.SCATTER_ROWS | L.ACCUMULATE, out_vals=acc)
+    torch.cuda.synchronize()
+    assert torch.equal(acc, 2.0 * v1)
+
+
 @pytest.mark.parametrize("cp", [False, True])
 @pytest.mark.parametrize("mode", ["f64", "mixed"])
 def test_hex64_pcg(hex64, mode, cp):

@@ -106,11 +106,19 @@ def test_tangent_csr(kind, etype, dt):
     assert np.array_equal(rp.cpu().numpy(), ip)
     assert np.array_equal(ci.cpu().numpy().astype(np.int64), ii)
     assert np.array_equal(A.indptr, ip) and np.array_equal(A.indices, ii)
-    for flags in (0, 0x20, 0x8):  # atomic, colour-ordered, transposed
+    for flags in (0, 0x20, 0x8, 0x80, 0x88):  # atomic, colour-ordered, transposed, row-owner (+ transposed)
         vals, R = ctx.tangent_csr(dev(case["field"], dt), dev(case["u"], dt), flags=flags)
         torch.cuda.synchronize()
         assert rel(vals.cpu().numpy(), A.data) <= TOL[dt], flags
         assert rel(R.cpu().numpy(), R_ref) <= TOL[dt], flags
+    # ACCUMULATE adds the assembly into existing values, in every assembly order
+    base = torch.linspace(-1.0, 1.0, A.data.size, dtype=torch.float64 if dt == "f64" else torch.float32,
+                          device="cuda")
+    for flags in (0x10, 0x30, 0x90):
+        out = base.clone()
+        ctx.tangent_csr(dev(case["field"], dt), dev(case["u"], dt), want_R=False, flags=flags, out_vals=out)
+        torch.cuda.synchronize()
+        assert rel(out.cpu().numpy(), base.cpu().numpy() + A.data) <= TOL[dt], flags
 
 
 @pytest.mark.parametrize("kind,etype", [("mechanical", "hexahedron"), ("thermal", "quad"), ("mechanical", "tetra")])

@@ -78,7 +78,8 @@ def test_forward_and_vjp_on_different_chunk_sizes_capture_in_one_graph():
 @pytest.mark.parametrize("kind", ["thermal", "mechanical"])
 def test_tangent_with_a_collapsed_element(kind):
     """A QUAD4 collapsed to a triangle (a node listed twice): the colour-ordered request assembles
-    with atomics for such meshes and equals the oracle's scipy assembly (duplicates summed)."""
+    with atomics for such meshes, and the row-owner assembly takes the element twice (once per local
+    index of the repeated node); both equal the oracle's scipy assembly (duplicates summed)."""
     from tests.cases import femhot_context, make_case
     case = make_case(kind, "quad", n=3, seed=63)
     conn = case["conn"].copy()
@@ -91,7 +92,7 @@ def test_tangent_with_a_collapsed_element(kind):
     A, R = fe.assemble(ph, case["coords"], conn, case["field"], case["u"], case["didx"])
     rp, ci = ctx.pattern()
     assert np.array_equal(rp.cpu().numpy(), A.indptr) and np.array_equal(ci.cpu().numpy(), A.indices)
-    for flags in (0x20, 0):
+    for flags in (0x20, 0, 0x80):
         vals, Rg = ctx.tangent_csr(f, u, flags=flags)
         torch.cuda.synchronize()
         assert rel(vals.cpu().numpy(), A.data) <= 1e-12, flags
