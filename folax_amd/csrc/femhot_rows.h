@@ -26,15 +26,130 @@
 
 namespace femhot {
 
+// Shape function value and local gradient of node a (unrolled callers make a a constant).
+template <int ET, class R>
+__device__ __forceinline__ void shape_node(int a, double x, double y, double z, R &Na, R (&dn)[ElemT<ET>::D]) {
+  if constexpr (ET == QUAD4) {
+    const double sx = (a == 1 || a == 2) ? 1.0 : -1.0, sy = (a >= 2) ? 1.0 : -1.0;
+    Na = R(0.25 * (1.0 + sx * x) * (1.0 + sy * y));
+    dn[0] = R(0.25 * sx * (1.0 + sy * y));
+    dn[1] = R(0.25 * sy * (1.0 + sx * x));
+  } else if constexpr (ET == HEX8) {
+    const int q = a & 3;
+    const double sx = (q == 1 || q == 2) ? 1.0 : -1.0, sy = (q >= 2) ? 1.0 : -1.0, sz = (a >= 4) ? 1.0 : -1.0;
+    const double px = 1.0 + sx * x, py = 1.0 + sy * y, pz = 1.0 + sz * z;
+    Na = R(0.125 * px * py * pz);
+    dn[0] = R(0.125 * sx * py * pz);
+    dn[1] = R(0.125 * sy * px * pz);
+    dn[2] = R(0.125 * sz * px * py);
+  } else {
+    Na = R(a == 0 ? 1.0 - (x + y + z) : a == 1 ? x : a == 2 ? y : z);
+    dn[0] = R(a == 0 ? -1.0 : a == 1 ? 1.0 : 0.0);
+    dn[1] = R(a == 0 ? -1.0 : a == 2 ? 1.0 : 0.0);
+    dn[2] = R(a == 0 ? -1.0 : a == 3 ? 1.0 : 0.0);
+  }
+}
+
+// Inverse Jacobian of Gauss point g of one element for the row-owner staging, with the shape function
+// values there: the arithmetic of gp_geometry (coordinates relative to node 0, J = sum_a x_a (x)
+// dN_a/dxi, inverse by cofactors), returning wdetJ.  xs: the element's node coordinates (N x 3, FP64).
+template <class R, int ET, int NGP>
+__device__ __forceinline__ R gp_inverse(int g, const double *xs, R (&Nv)[ElemT<ET>::N],
+                                        R (&inv)[ElemT<ET>::D][ElemT<ET>::D]) {
+  constexpr int N = ElemT<ET>::N, D = ElemT<ET>::D;
+  double xi, eta, zeta, w;
+  gauss_point<ET, NGP>(g, xi, eta, zeta, w);
+  double x0[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) x0[i] = xs[i];
+  R J[D][D];
+  {
+    R dn[D];
+    shape_node<ET, R>(0, xi, eta, zeta, Nv[0], dn);
+  }
+#pragma unroll
+  for (int a = 1; a < N; ++a) {
+    R xa[D], dn[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) xa[i] = R(xs[a * 3 + i] - x0[i]);
+    shape_node<ET, R>(a, xi, eta, zeta, Nv[a], dn);
+    if constexpr (ET == TET4) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) J[i][a - 1] = xa[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) J[i][k] = (a == 1) ? xa[i] * dn[k] : J[i][k] + xa[i] * dn[k];
+    }
+  }
+  R det;
+  if constexpr (D == 2) {
+    det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const R id = recip(det);
+    inv[0][0] = J[1][1] * id;
+    inv[0][1] = -J[0][1] * id;
+    inv[1][0] = -J[1][0] * id;
+    inv[1][1] = J[0][0] * id;
+  } else {
+    const R c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const R c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const R c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const R id = recip(det);
+    inv[0][0] = c00 * id;
+    inv[1][0] = c01 * id;
+    inv[2][0] = c02 * id;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  }
+  return R(w) * det;
+}
+
+// dN_b/dx = dN_b/dxi . inv(J) of every node at Gauss point g, stored at st[b * D + j] (TET4: the
+// rows of inv(J), node 0 minus their sum, as gp_geometry).
+template <class R, int ET, int NGP>
+__device__ __forceinline__ void stage_gradients(int g, const R (&inv)[ElemT<ET>::D][ElemT<ET>::D], R *st) {
+  constexpr int N = ElemT<ET>::N, D = ElemT<ET>::D;
+  if constexpr (ET == TET4) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      st[1 * D + j] = inv[0][j];
+      st[2 * D + j] = inv[1][j];
+      st[3 * D + j] = inv[2][j];
+      st[j] = -((inv[0][j] + inv[1][j]) + inv[2][j]);
+    }
+  } else {
+    double xi, eta, zeta, w;
+    gauss_point<ET, NGP>(g, xi, eta, zeta, w);
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      R Nb, dn[D];
+      shape_node<ET, R>(b, xi, eta, zeta, Nb, dn);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        R s = dn[0] * inv[0][j];
+#pragma unroll
+        for (int k = 1; k < D; ++k) s += dn[k] * inv[k][j];
+        st[b * D + j] = s;
+      }
+    }
+  }
+}
+
 template <int ET, int NGP, int N, int P> struct RowsShape {
   static constexpr int G = RuleSize<ET, NGP>::G;
   static constexpr int EB = (64 / G < 64 / N) ? 64 / G : 64 / N;  // incident elements per batch
-  static constexpr int KW = (P == 3) ? 56 : 64;                    // neighbour columns per window
+  static constexpr int KW = (P == 3) ? 28 : 64;                    // neighbour columns per window
   static constexpr int OR = (P * P * KW + 63) / 64;                // output values per lane per window
 };
 
 template <class R, int ET, int PH, int NGP>
-__global__ __launch_bounds__(64) void k_tangent_rows(TArgs A) {
+__global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
   static_assert(PH != NEOHOOKE, "row-owner assembly covers thermal and linear elasticity");
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, P = E::P;
@@ -44,6 +159,8 @@ __global__ __launch_bounds__(64) void k_tangent_rows(TArgs A) {
   constexpr int ST = N * D + 1;  // staged values per (element, Gauss point): dN/dx of N nodes, f_g
   constexpr int SW = (EB * G * ST > 64 * P * P) ? EB * G * ST : 64 * P * P;
   __shared__ R S[SW];  // staged geometry, then (aliased) the 64 lanes' P x P blocks
+  __shared__ double XS[64 * 3];  // node coordinates of the batch's elements, lane (s, b) -> node b of s
+  __shared__ R FS[64], US[(PH == THERMAL) ? 64 : 1];  // field (and thermal solution) at those nodes
   __shared__ unsigned long long MK[KW];
   const int lane = threadIdx.x;
   const long long node = blockIdx.x;
@@ -58,6 +175,8 @@ __global__ __launch_bounds__(64) void k_tangent_rows(TArgs A) {
   const R *F = reinterpret_cast<const R *>(A.field);
   const R *U = reinterpret_cast<const R *>(A.u);
   R *V = reinterpret_cast<R *>(A.vals) + rp;
+  const int s2 = lane / N, b2 = lane - s2 * N;  // lane (element, node) of steps 0 and 2
+  const int s1 = lane / G, g1 = lane - s1 * G;  // lane (element, Gauss point) of step 1
   for (int k0 = 0; k0 < deg; k0 += KW) {
     const int kw = min(KW, deg - k0), nout = P * P * kw;
     // this lane's outputs in the window: t = r * 64 + lane -> row c, column k0 + k, component d
@@ -74,101 +193,115 @@ __global__ __launch_bounds__(64) void k_tangent_rows(TArgs A) {
     for (int r = 0; r < OR; ++r) acc[r] = R(0);
     for (int base = 0; base < cnt; base += EB) {
       const int nb = min(EB, cnt - base);
-      {  // 1. geometry of (element, Gauss point) pairs
-        const int s = lane / G, g = lane - s * G;
-        if (s < nb) {
-          const int e = A.ne_list[off0 + base + s] / N;
-          int nodes[N];
-          load_conn<N>(A.conn, e, nodes);
-          E el;
-          load_geometry<R, ET, PH>(A.coords, nodes, el);
-          GP<R, ET> gp;
-          gp_geometry<R, ET, NGP>(g, el.x, gp);
-          R eg;
-          if (elemental) {
-            eg = F[e];
-          } else {
-            eg = R(-0.0);
+      // 0. lane (s, b) fetches node b of incident element s: coordinates, field and (thermal)
+      //    solution to LDS; its column among the owned node's neighbours stays in a register
+      int a2 = 0, col = -1;
+      if (s2 < nb) {
+        const int ea = A.ne_list[off0 + base + s2];
+        const int e = ea / N;
+        a2 = ea - e * N;
+        col = A.slot[((size_t)e * N + a2) * N + b2];
+        const int nd = A.conn[(size_t)e * N + b2];
 #pragma unroll
-            for (int b = 0; b < N; ++b) eg += gp.Nv[b] * F[nodes[b]];
-          }
-          R f = gp.wdet * eg;
-          if constexpr (PH == THERMAL) {
-            R Tg = R(-0.0);
+        for (int i = 0; i < 3; ++i) XS[lane * 3 + i] = A.coords[(size_t)nd * 3 + i];
+        FS[lane] = elemental ? F[e] : F[nd];
+        if constexpr (PH == THERMAL) US[lane] = U[nd];
+      }
+      __syncthreads();
+      if (s1 < nb) {  // 1. geometry of (element, Gauss point) pairs
+        const int n0 = s1 * N;
+        R *st = S + lane * ST;
+        R Nv[N], inv[D][D];
+        const R wdet = gp_inverse<R, ET, NGP>(g1, XS + n0 * 3, Nv, inv);
+        R eg;
+        if (elemental) {
+          eg = FS[n0];
+        } else {
+          eg = R(-0.0);
 #pragma unroll
-            for (int b = 0; b < N; ++b) Tg += gp.Nv[b] * U[nodes[b]];
-            f *= R(1) + R(A.pp.beta) * powc(Tg, R(A.pp.c));
-          }
-          R *st = S + lane * ST;
-#pragma unroll
-          for (int b = 0; b < N; ++b)
-#pragma unroll
-            for (int j = 0; j < D; ++j) st[b * D + j] = gp.DN[b][j];
-          st[N * D] = f;
+          for (int b = 0; b < N; ++b) eg += Nv[b] * FS[n0 + b];
         }
+        R f = wdet * eg;
+        if constexpr (PH == THERMAL) {
+          R Tg = R(-0.0);
+#pragma unroll
+          for (int b = 0; b < N; ++b) Tg += Nv[b] * US[n0 + b];
+          f *= R(1) + R(A.pp.beta) * powc(Tg, R(A.pp.c));
+        }
+        // inv(J) and f make a round trip through this lane's own staging slot, and the Gauss point
+        // index an opaque copy, so the compiler keeps neither the Jacobian step's values nor its
+        // shape derivatives live into the gradient step (fewer registers, more waves per SIMD)
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+          for (int j = 0; j < D; ++j) st[i * D + j] = inv[i][j];
+        st[N * D] = f;
+        asm volatile("" ::: "memory");
+        int go = g1;
+        asm volatile("" : "+v"(go));
+        R iv[D][D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+          for (int j = 0; j < D; ++j) iv[i][j] = st[i * D + j];
+        stage_gradients<R, ET, NGP>(go, iv, st);
       }
       __syncthreads();
       R K[P * P];
-      int col = -1;
-      {  // 2. block (a, b) of each incident element's tangent
-        const int s = lane / N, b = lane - s * N;
 #pragma unroll
-        for (int i = 0; i < P * P; ++i) K[i] = R(0);
-        if (s < nb) {
-          const int ea = A.ne_list[off0 + base + s];
-          const int e = ea / N, a = ea - e * N;
-          col = A.slot[((size_t)e * N + a) * N + b];
-          R M[D][D];
+      for (int i = 0; i < P * P; ++i) K[i] = R(0);
+      if (s2 < nb) {  // 2. block (a, b) of each incident element's tangent
+        const int a = a2, b = b2;
+        R M[D][D];
 #pragma unroll
-          for (int p = 0; p < D; ++p)
+        for (int p = 0; p < D; ++p)
 #pragma unroll
-            for (int q = 0; q < D; ++q) M[p][q] = R(-0.0);
-          R t = R(-0.0);
+          for (int q = 0; q < D; ++q) M[p][q] = R(-0.0);
+        R t = R(-0.0);
 #pragma unroll UNR
-          for (int g = 0; g < G; ++g) {
-            const R *st = S + (s * G + g) * ST;
-            const R f = st[N * D];
-            R fa[D], db[D];
+        for (int g = 0; g < G; ++g) {
+          const R *st = S + (s2 * G + g) * ST;
+          const R f = st[N * D];
+          R fa[D], db[D];
 #pragma unroll
-            for (int j = 0; j < D; ++j) {
-              fa[j] = f * st[a * D + j];
-              db[j] = st[b * D + j];
-            }
-            if constexpr (PH == THERMAL) {
-#pragma unroll
-              for (int j = 0; j < D; ++j) t += fa[j] * db[j];
-            } else {
-#pragma unroll
-              for (int p = 0; p < D; ++p)
-#pragma unroll
-                for (int q = 0; q < D; ++q) M[p][q] += fa[p] * db[q];
-            }
+          for (int j = 0; j < D; ++j) {
+            fa[j] = f * st[a * D + j];
+            db[j] = st[b * D + j];
           }
           if constexpr (PH == THERMAL) {
-            K[0] = t;
+#pragma unroll
+            for (int j = 0; j < D; ++j) t += fa[j] * db[j];
           } else {
-            const R c2 = R(D == 3 ? A.pp.c2 : A.pp.f2), c3 = R(D == 3 ? A.pp.c3 : A.pp.f2nu),
-                    c4 = R(D == 3 ? A.pp.c4 : A.pp.f2sh);
 #pragma unroll
-            for (int c = 0; c < D; ++c)
+            for (int p = 0; p < D; ++p)
 #pragma unroll
-              for (int d = 0; d < D; ++d) {
-                if (c == d) {
-                  R o = M[(c + 1) % D][(c + 1) % D];
-                  if constexpr (D == 3) o += M[(c + 2) % D][(c + 2) % D];
-                  K[c * P + d] = c2 * M[c][c] + c4 * o;
-                } else {
-                  K[c * P + d] = c3 * M[c][d] + c4 * M[d][c];
-                }
-              }
+              for (int q = 0; q < D; ++q) M[p][q] += fa[p] * db[q];
           }
-          // a Dirichlet row keeps only the element diagonal (fe_loss.py:191-207)
-#pragma unroll
-          for (int c = 0; c < P; ++c)
-#pragma unroll
-            for (int d = 0; d < P; ++d)
-              if (dm[c] && !(b == a && d == c)) K[c * P + d] = R(0);
         }
+        if constexpr (PH == THERMAL) {
+          K[0] = t;
+        } else {
+          const R c2 = R(D == 3 ? A.pp.c2 : A.pp.f2), c3 = R(D == 3 ? A.pp.c3 : A.pp.f2nu),
+                  c4 = R(D == 3 ? A.pp.c4 : A.pp.f2sh);
+#pragma unroll
+          for (int c = 0; c < D; ++c)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              if (c == d) {
+                R o = M[(c + 1) % D][(c + 1) % D];
+                if constexpr (D == 3) o += M[(c + 2) % D][(c + 2) % D];
+                K[c * P + d] = c2 * M[c][c] + c4 * o;
+              } else {
+                K[c * P + d] = c3 * M[c][d] + c4 * M[d][c];
+              }
+            }
+        }
+        // a Dirichlet row keeps only the element diagonal (fe_loss.py:191-207)
+#pragma unroll
+        for (int c = 0; c < P; ++c)
+#pragma unroll
+          for (int d = 0; d < P; ++d)
+            if (dm[c] && !(b == a && d == c)) K[c * P + d] = R(0);
       }
       __syncthreads();  // every lane has read the staged geometry: the space takes the blocks
       for (int t = lane; t < kw; t += 64) MK[t] = 0ull;
@@ -182,10 +315,20 @@ __global__ __launch_bounds__(64) void k_tangent_rows(TArgs A) {
         if (ok[r] < 0) continue;
         unsigned long long m = MK[ok[r]];
         R sum = R(0);
-        while (m) {
-          const int j = __builtin_ctzll(m);
-          m &= m - 1;
-          sum += S[j * P * P + ocd[r]];
+        while (m) {  // four contributions per round, their loads issued together
+          int j[4];
+          bool v[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            v[q] = m != 0;
+            j[q] = v[q] ? __builtin_ctzll(m) : 0;
+            m &= m - 1;
+          }
+          R x[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) x[q] = S[j[q] * P * P + ocd[r]];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) sum += v[q] ? x[q] : R(0);
         }
         acc[r] += sum;
       }
