@@ -158,10 +158,18 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
   constexpr int UNR = G <= 8 ? G : 1;
   constexpr int ST = N * D + 1;  // staged values per (element, Gauss point): dN/dx of N nodes, f_g
   constexpr int SW = (EB * G * ST > 64 * P * P) ? EB * G * ST : 64 * P * P;
-  __shared__ R S[SW];  // staged geometry, then (aliased) the 64 lanes' P x P blocks
-  __shared__ double XS[64 * 3];  // node coordinates of the batch's elements, lane (s, b) -> node b of s
-  __shared__ R FS[64], US[(PH == THERMAL) ? 64 : 1];  // field (and thermal solution) at those nodes
+  // LDS: S holds the staged geometry, then the 64 lanes' P x P blocks.  The node data of step 0
+  // (coordinates XS, field FS, thermal solution US of lane (s, b) = node b of incident element s)
+  // sits at the end of the same space: step 1 reads all of it before it writes the geometry (each
+  // lane's writes depend on its reads, and the workgroup is one wave), so the two never overlap in
+  // time and the workgroup needs no extra LDS.
+  constexpr int XW = (64 * 3 * 8 + 2 * 64 * (int)sizeof(R)) / (int)sizeof(R);  // a multiple of 4
+  constexpr int SX = ((SW > XW ? SW : XW) + 3) & ~3;  // so XS starts 16-byte aligned
+  __shared__ __align__(16) R S[SX];
   __shared__ unsigned long long MK[KW];
+  double *XS = reinterpret_cast<double *>(S + SX - XW);
+  R *FS = reinterpret_cast<R *>(XS + 64 * 3);
+  R *US = FS + 64;
   const int lane = threadIdx.x;
   const long long node = blockIdx.x;
   const int off0 = A.ne_off[node], cnt = A.ne_off[node + 1] - off0;
