@@ -167,6 +167,7 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
   constexpr int SX = ((SW > XW ? SW : XW) + 3) & ~3;  // so XS starts 16-byte aligned
   __shared__ __align__(16) R S[SX];
   __shared__ unsigned long long MK[KW];
+  __shared__ int CS[KW + 1];
   double *XS = reinterpret_cast<double *>(S + SX - XW);
   R *FS = reinterpret_cast<R *>(XS + 64 * 3);
   R *US = FS + 64;
@@ -215,7 +216,10 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
         FS[lane] = elemental ? F[e] : F[nd];
         if constexpr (PH == THERMAL) US[lane] = U[nd];
       }
+      for (int t = lane; t < kw; t += 64) MK[t] = 0ull;
       __syncthreads();
+      // the lanes of each column of the window, as a bit mask
+      if (col >= k0 && col < k0 + kw) atomicOr(&MK[col - k0], 1ull << lane);
       if (s1 < nb) {  // 1. geometry of (element, Gauss point) pairs
         const int n0 = s1 * N;
         R *st = S + lane * ST;
@@ -255,6 +259,16 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
         stage_gradients<R, ET, NGP>(go, iv, st);
       }
       __syncthreads();
+      {  // blocks are stored column by column, lanes in order: CS[k] = first block of column k
+        int pc = lane < kw ? __popcll(MK[lane]) : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int y = __shfl_up(pc, o);
+          if (lane >= o) pc += y;
+        }
+        if (lane < kw) CS[lane + 1] = pc;
+        if (lane == 0) CS[0] = 0;
+      }
       R K[P * P];
 #pragma unroll
       for (int i = 0; i < P * P; ++i) K[i] = R(0);
@@ -312,31 +326,24 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
             if (dm[c] && !(b == a && d == c)) K[c * P + d] = R(0);
       }
       __syncthreads();  // every lane has read the staged geometry: the space takes the blocks
-      for (int t = lane; t < kw; t += 64) MK[t] = 0ull;
+      if (col >= k0 && col < k0 + kw) {
+        const int k = col - k0;
+        const int dst = CS[k] + __popcll(MK[k] & ((1ull << lane) - 1ull));
 #pragma unroll
-      for (int i = 0; i < P * P; ++i) S[lane * P * P + i] = K[i];
-      __syncthreads();
-      if (col >= k0 && col < k0 + kw) atomicOr(&MK[col - k0], 1ull << lane);
+        for (int i = 0; i < P * P; ++i) S[dst * P * P + i] = K[i];
+      }
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < OR; ++r) {  // 3. each output sums its column's blocks in lane order
         if (ok[r] < 0) continue;
-        unsigned long long m = MK[ok[r]];
+        const int lo = CS[ok[r]], hi = CS[ok[r] + 1];
         R sum = R(0);
-        while (m) {  // four contributions per round, their loads issued together
-          int j[4];
-          bool v[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            v[q] = m != 0;
-            j[q] = v[q] ? __builtin_ctzll(m) : 0;
-            m &= m - 1;
-          }
+        for (int i = lo; i < hi; i += 4) {  // four contributions per round, their loads issued together
           R x[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) x[q] = S[j[q] * P * P + ocd[r]];
+          for (int q = 0; q < 4; ++q) x[q] = S[min(i + q, hi - 1) * P * P + ocd[r]];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) sum += v[q] ? x[q] : R(0);
+          for (int q = 0; q < 4; ++q) sum += (i + q < hi) ? x[q] : R(0);
         }
         acc[r] += sum;
       }
