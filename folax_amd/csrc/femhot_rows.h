@@ -179,6 +179,9 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
   bool dm[P];
 #pragma unroll
   for (int c = 0; c < P; ++c) dm[c] = A.dmask[node * P + c] != 0;
+  bool any_dm = false;
+#pragma unroll
+  for (int c = 0; c < P; ++c) any_dm = any_dm || dm[c];
   const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
   const bool accumulate = (A.flags & FEMHOT_ACCUMULATE) != 0;
   const R *F = reinterpret_cast<const R *>(A.field);
@@ -188,14 +191,25 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
   const int s1 = lane / G, g1 = lane - s1 * G;  // lane (element, Gauss point) of step 1
   for (int k0 = 0; k0 < deg; k0 += KW) {
     const int kw = min(KW, deg - k0), nout = P * P * kw;
-    // this lane's outputs in the window: t = r * 64 + lane -> row c, column k0 + k, component d
-    int ok[OR], ocd[OR];
+    // this lane's outputs in the window: t = r * 64 + lane -> row c, column k0 + k, component d, at
+    // offset oo in the node's P rows (row c starts at c * P * deg); no integer division by kw
+    int ok[OR], ocd[OR], oo[OR];
+    const int pk = P * kw, pdeg = P * deg;
 #pragma unroll
     for (int r = 0; r < OR; ++r) {
       const int t = r * 64 + lane;
-      const int c = t / (P * kw), rem = t - c * (P * kw), k = rem / P;
+      int c = 0, cs = 0, cr = 0;
+#pragma unroll
+      for (int q = 1; q < P; ++q) {
+        const bool ge = t >= cs + pk;
+        c += ge ? 1 : 0;
+        cs += ge ? pk : 0;
+        cr += ge ? pdeg : 0;
+      }
+      const int rem = t - cs, k = rem / P;
       ok[r] = t < nout ? k : -1;
       ocd[r] = c * P + (rem - k * P);
+      oo[r] = cr + k0 * P + rem;
     }
     R acc[OR];
 #pragma unroll
@@ -318,12 +332,15 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
               }
             }
         }
-        // a Dirichlet row keeps only the element diagonal (fe_loss.py:191-207)
+        // a Dirichlet row keeps only the element diagonal (fe_loss.py:191-207); dm is the same for
+        // the whole workgroup, so free nodes skip this
+        if (any_dm) {
 #pragma unroll
-        for (int c = 0; c < P; ++c)
+          for (int c = 0; c < P; ++c)
 #pragma unroll
-          for (int d = 0; d < P; ++d)
-            if (dm[c] && !(b == a && d == c)) K[c * P + d] = R(0);
+            for (int d = 0; d < P; ++d)
+              if (dm[c] && !(b == a && d == c)) K[c * P + d] = R(0);
+        }
       }
       __syncthreads();  // every lane has read the staged geometry: the space takes the blocks
       if (col >= k0 && col < k0 + kw) {
@@ -352,8 +369,7 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
 #pragma unroll
     for (int r = 0; r < OR; ++r) {
       if (ok[r] < 0) continue;
-      const int c = ocd[r] / P, d = ocd[r] - c * P;
-      R *v = V + (long long)c * P * deg + (long long)(k0 + ok[r]) * P + d;
+      R *v = V + oo[r];
       *v = accumulate ? *v + acc[r] : acc[r];
     }
   }
