@@ -15,8 +15,10 @@ per-mesh potentials, for all 32 meshes:
     mesh, 32 calls per step.
 value = batched FP32 Melem/s; also reported: the plain FP64 rate and the ratio, the FP32 error
 against FP64, the roofline of the dominant kernels, the FP64 C/OpenMP CPU baseline (all host cores
-and one), and a second workload: nonlinear thermal QUAD4 128^2 x 64 samples, the residual-norm loss
-and its VJP with respect to T and k0 (FP32, one graph replay per step).
+and one), and two more workloads: nonlinear thermal QUAD4 128^2 x 64 samples, the residual-norm loss
+and its VJP with respect to T and k0 (FP32, one graph replay per step); and HEX8 64^3 linear
+elasticity, BC application + Jacobian (CSR tangent) + residual in FP64, the call the reference's own
+timing harness times (examples/performance_study/matrix_vector_perf.py).
 """
 import argparse
 import json
@@ -33,6 +35,7 @@ METRIC = "batched FP32 residual+VJP Melem/s vs plain FP64 at ≤1e-5 rel err; % 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "tet4_uneven_chunk_traffic.json")
 C3_TRAFFIC_JSON = os.path.join(ROOT, "profiles", "quad128x64_thermal_traffic.json")
+C2_TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hex64_tangent_traffic.json")
 
 
 def log(*a):
@@ -271,6 +274,79 @@ def run_c3(args, torch, L):
     return res
 
 
+def cpu_baseline_c2(n=20):
+    """The oracle's FP64 assembly (numpy, one process) of the reference timing harness's own problem
+    (examples/performance_study/matrix_vector_perf.py: HEX8 20^3 box, E = 1, BCs on left/right)."""
+    from oracle import fe
+    coords, conn, sets = fe.box_hex_mesh(n, n, n)
+    bc = {"Ux": {"left": 0.0, "right": 0.1}, "Uy": {"left": 0.0, "right": 0.1}, "Uz": {"left": 0.0, "right": 0.1}}
+    didx, dval = fe.dirichlet_dofs(sets, ["Ux", "Uy", "Uz"], bc)
+    ph = fe.Physics("mechanical", "hexahedron", num_gp=2, E=1.0, nu=0.3)
+    u = fe.apply_dirichlet(np.zeros(3 * coords.shape[0]), didx, dval)
+    t0 = time.time()
+    fe.assemble(ph, coords, conn, np.ones(coords.shape[0]), u, didx)
+    dt = time.time() - t0
+    return {"value": round(conn.shape[0] / dt / 1e6, 5), "unit": "Melem/s", "cores": 1, "kind": "port",
+            "cpu": cpu_model(), "sample": f"oracle/fe.py assemble (numpy FP64, tangent + residual) of the HEX8 "
+                                         f"{n}^3 harness problem, one call, {dt:.1f} s"}
+
+
+def run_c2(args, torch, L):
+    """HEX8 64^3 linear elasticity (E in {1, 10} from the two-phase random field, clamped at z = 0, 1 %
+    stretch at z = 1), FP64: BC application + ComputeJacobianMatrixAndResidualVector through
+    MechanicalLoss3DHexa, as the reference's timing harness times it (matrix_vector_perf.py:50-69), and
+    the tangent assembly alone in each of its three orders."""
+    from folax_amd import loss_functions as lf
+    from folax_amd.controls import grf_two_phase
+    from folax_amd.mesh import create_3D_box_mesh
+    t0 = time.time()
+    m = create_3D_box_mesh(64, 64, 64)
+    bc = {"Ux": {"bottom": 0.0}, "Uy": {"bottom": 0.0}, "Uz": {"bottom": 0.0, "top": 0.01}}
+    loss = lf.MechanicalLoss3DHexa("c2", {"dirichlet_bc_dict": bc, "num_gp": 2,
+                                          "material_dict": {"young_modulus": 1.0, "poisson_ratio": 0.3}}, m)
+    loss.Initialize()
+    dev = torch.device("cuda")
+    Nn, Ne = m.GetNumberOfNodes(), m.GetNumberOfElements("hexahedron")
+    E = torch.as_tensor(grf_two_phase((65, 65, 65), 8.0, 2001), dtype=torch.float64, device=dev)
+    u0 = torch.zeros(3 * Nn, dtype=torch.float64, device=dev)
+
+    def call():
+        u = loss.ApplyDirichletBCOnDofVector(u0)
+        return loss.ComputeJacobianMatrixAndResidualVector(E, u)
+
+    K, R = call()  # pattern, plans and first launch
+    torch.cuda.synchronize()
+    log(f"C2 set-up {time.time() - t0:.1f}s, nnz {K.values.numel()}")
+    reps = max(5, min(args.steps, 20))
+    call_ms = time_replays(call, reps, torch)
+    ctx, u = loss.ctx, loss.ApplyDirichletBCOnDofVector(u0)
+    vals = torch.empty_like(K.values)
+    t = {}
+    for name, fl in (("rows", L.SCATTER_ROWS), ("colour", L.SCATTER_COLOUR), ("atomic", 0)):
+        ctx.tangent_csr(E, u, want_R=False, flags=fl, out_vals=vals)
+        t[name] = time_replays(lambda: ctx.tangent_csr(E, u, want_R=False, flags=fl, out_vals=vals),
+                               reps if name == "rows" else 3, torch)
+    nnz = K.values.numel()
+    # compulsory bytes of one tangent assembly: the CSR values written once, the mesh (connectivity,
+    # coordinates) and the nodal E read once
+    alg = nnz * 8 + Ne * 32 + Nn * (24 + 8)
+    prof = load_profiled(C2_TRAFFIC_JSON) or {}
+    res = {"workload": "HEX8 64^3 linear elasticity, two-phase E, FP64: ApplyDirichletBCOnDofVector + "
+                       "ComputeJacobianMatrixAndResidualVector (CSR tangent + masked residual)",
+           "elements": Ne, "nnz": nnz, "ms_per_call": round(call_ms, 4),
+           "value": round(Ne / (call_ms * 1e-3) / 1e6, 2), "unit": "Melem/s",
+           "tangent_ms": {k: round(v, 4) for k, v in t.items()},
+           "algorithmic_bytes_tangent": int(alg),
+           "roofline": roofline(alg, t["rows"], prof.get("bytes_per_launch") if prof.get("path") == "rows" else None)}
+    log(f"C2: call {call_ms * 1e3:.0f} us, tangent rows/colour/atomic {t['rows'] * 1e3:.0f} / "
+        f"{t['colour'] * 1e3:.0f} / {t['atomic'] * 1e3:.0f} us")
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_c2()
+        log(f"C2 CPU baseline: {res['cpu_baseline']}")
+    loss.ctx.close()
+    return res
+
+
 def solver_profile(iters=200):
     """HEX8 64^3 linear elasticity (274,625 nodes, 823,875 dofs), E in {1, 10} from a seeded Gaussian random
     field (correlation length 8 voxels), clamped at z = 0 with 1 % stretch at z = 1: FP64 tangent + residual
@@ -345,6 +421,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--no-c3", action="store_true", help="skip the QUAD4 128^2 x 64 thermal line")
+    ap.add_argument("--no-c2", action="store_true", help="skip the HEX8 64^3 Jacobian + residual line")
+    ap.add_argument("--c2-only", action="store_true", help="run only the HEX8 64^3 Jacobian line (for rocprofv3)")
     ap.add_argument("--autotune", action="store_true", help="autotune instead of the profiled configuration")
     ap.add_argument("--profile", choices=["c5", "c3", "c4"], default=None,
                     help="short run of one workload's kernels for rocprofv3 (c5: + a 256 MiB copy for calibration)")
@@ -371,6 +449,10 @@ def main():
         return solver_profile()
     if args.profile == "c4":
         return c4_profile(args)
+    if args.c2_only:
+        args.no_cpu = True
+        print(json.dumps(run_c2(args, torch, L)), flush=True)
+        return
     if args.profile == "c3":
         args.no_cpu = True
         run_c3(args, torch, L)
@@ -477,6 +559,7 @@ def main():
         err["cpu_fp64_vs_gpu_fp64_g_u"] = rel(gu64, cpu_out[0])
         log(f"CPU baseline: {cpu}")
     c3 = None if args.no_c3 else run_c3(args, torch, L)
+    c2 = None if args.no_c2 else run_c2(args, torch, L)
     value = Ne / (step_ms * 1e-3) / 1e6
     v64 = Ne / (step64_ms * 1e-3) / 1e6
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Melem/s", "n_gpus": 1, "steps": args.steps,
@@ -493,7 +576,7 @@ def main():
            "speedup_vs_fp64_plain": round(value / v64, 2),
            "rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()},
            "kernel_ms": round(kernel_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes),
-           "roofline": roof, "cpu_baseline": cpu, "c3_quad128x64_thermal": c3}
+           "roofline": roof, "cpu_baseline": cpu, "c3_quad128x64_thermal": c3, "c2_hex64_jacobian": c2}
     print(json.dumps(out), flush=True)
 
 

@@ -172,7 +172,12 @@ __global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
   R *FS = reinterpret_cast<R *>(XS + 64 * 3);
   R *US = FS + 64;
   const int lane = threadIdx.x;
-  const long long node = blockIdx.x;
+  // Workgroups go to the 8 XCDs in turn, and nodes with nearby ids share elements: XCD x takes the
+  // x-th eighth of the nodes in order, so its workgroups find their neighbours' element data in
+  // its own L2 (a bijection of workgroup ids onto nodes)
+  const long long per = gridDim.x / 8, rem = gridDim.x % 8;
+  const long long xcd = blockIdx.x % 8, q = blockIdx.x / 8;
+  const long long node = xcd * per + (xcd < rem ? xcd : rem) + q;
   const int off0 = A.ne_off[node], cnt = A.ne_off[node + 1] - off0;
   const long long rp = A.row_ptr[node * P];
   const int deg = (int)((A.row_ptr[node * P + 1] - rp) / P);

@@ -214,7 +214,8 @@ class FiniteElementLoss:
         u = _as_device_tensor(total_primal_vars)
         d = _as_device_tensor(total_control_vars, dtype=u.dtype)
         # "chunk" leaves the assembly order to the library: the path femhot_autotune measured for the
-        # tangent (with autotune on), else colour-ordered (deterministic); "atomic": hardware atomics
+        # tangent (with autotune on), else row-owner assembly (thermal, linear elastic) or colour-ordered
+        # adds (Neo-Hookean), both deterministic; "atomic": hardware atomics
         flags = self._flags() | (L.TRANSPOSE if transpose_jacobian else 0)
         if self.scatter == "chunk" and self.autotune and (L.OP_TANGENT, u.dtype, 1) not in self._tuned:
             self.ctx.autotune_tangent(d, u, flags=flags & ~L.SCATTER_CHUNK)
