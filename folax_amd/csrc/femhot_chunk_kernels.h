@@ -129,7 +129,18 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   const int tid = threadIdx.x;
   const int n_items = A.n_chunks * A.n_sblocks;
   using UN = typename std::conditional<N == 4, uint2, uint4>::type;  // one element's N uint16
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+  // Workgroups go to the 8 XCDs in turn; neighbouring chunks share nodes (their solution gathers and
+  // the partial slots of a shared node, which sit side by side), so XCD x walks the x-th eighth of
+  // the items in order and finds those lines in its own L2.  Items [i0, i1) of XCD x are split over
+  // its nx workgroups with stride nx (every item is taken exactly once).
+  // (Grids of fewer than 8 workgroups walk all items with the plain stride.)
+  const bool xmap = gridDim.x >= 8;
+  const int xcd = blockIdx.x % 8, nx = gridDim.x / 8 + (xcd < (int)(gridDim.x % 8) ? 1 : 0);
+  const int ipx = n_items / 8, irem = n_items % 8;
+  const int i0 = xmap ? xcd * ipx + min(xcd, irem) : 0;
+  const int i1 = xmap ? i0 + ipx + (xcd < irem ? 1 : 0) : n_items;
+  const int istride = xmap ? nx : (int)gridDim.x;
+  for (int item = i0 + (int)(xmap ? blockIdx.x / 8 : blockIdx.x); item < i1; item += istride) {
     const int chunk = item / A.n_sblocks;
     const int sbi = item - chunk * A.n_sblocks;
     const int e0 = A.elem_off[chunk], e1 = A.elem_off[chunk + 1];
