@@ -331,7 +331,9 @@ int femhot_tangent_csr(femhot_ctx *h, int dtype, const void *field, const void *
     io.field = field;
     io.u = u;
     io.ou = R;
-    io.flags = (flags & (FEMHOT_ELEMENT_FIELD | FEMHOT_ACCUMULATE | FEMHOT_SCATTER_COLOUR)) | FEMHOT_MASK_DIRICHLET;
+    // the residual takes the scatter the call asks for (FEMHOT_SCATTER_CHUNK: its tuned path)
+    io.flags = (flags & (FEMHOT_ELEMENT_FIELD | FEMHOT_ACCUMULATE | FEMHOT_SCATTER_COLOUR | FEMHOT_SCATTER_CHUNK)) |
+               FEMHOT_MASK_DIRICHLET;
     io.stream = s;
     if ((rc = run_elem(c, io))) return rc;
   }
