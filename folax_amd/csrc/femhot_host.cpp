@@ -88,6 +88,14 @@ int ensure_pattern(Ctx *c) {
         for (int d = 0; d < P; ++d) c->h_col_idx[r++] = adj[adj_off[i] + k] * P + d;
     }
   }
+  // diagonal of each row: offset within the row of column (i, c) for row (i, c), -1 for an empty row
+  std::vector<int> diag_off(c->Ndof);
+  for (long long i = 0; i < Nn; ++i) {
+    const int *beg = adj.data() + adj_off[i], *end = adj.data() + adj_off[i + 1];
+    const int *at = std::lower_bound(beg, end, (int)i);  // a node in some element is its own neighbour
+    const bool has = at != end && *at == (int)i;         // (a node in no element has an empty row)
+    for (int cc = 0; cc < P; ++cc) diag_off[i * P + cc] = has ? (int)((at - beg) * P + cc) : -1;
+  }
   // slot table: position of node b in the sorted neighbour list of node a, per element
   std::vector<uint8_t> slot((size_t)Ne * n * n);
   for (long long e = 0; e < Ne; ++e)
@@ -110,6 +118,8 @@ int ensure_pattern(Ctx *c) {
   FH_HIP(hipMemcpy(c->d_row_ptr, c->h_row_ptr.data(), sizeof(long long) * (c->Ndof + 1), hipMemcpyHostToDevice));
   if (base) FH_HIP(hipMemcpy(c->d_col_idx, c->h_col_idx.data(), sizeof(int) * base, hipMemcpyHostToDevice));
   FH_HIP(hipMemcpy(c->d_slot, slot.data(), slot.size(), hipMemcpyHostToDevice));
+  FH_HIP(hipMalloc(&c->d_diag_off, sizeof(int) * std::max<long long>(1, c->Ndof)));
+  if (c->Ndof) FH_HIP(hipMemcpy(c->d_diag_off, diag_off.data(), sizeof(int) * c->Ndof, hipMemcpyHostToDevice));
   FH_HIP(hipMalloc(&c->d_ne_off, sizeof(int) * (Nn + 1)));
   FH_HIP(hipMalloc(&c->d_ne_list, sizeof(int) * std::max<size_t>(1, ne_list.size())));
   FH_HIP(hipMemcpy(c->d_ne_off, ne_off.data(), sizeof(int) * (Nn + 1), hipMemcpyHostToDevice));
@@ -680,6 +690,7 @@ void femhot_ctx_destroy(femhot_ctx *h) {
   (void)hipFree(c.d_col_idx);
   (void)hipFree(c.d_slot);
   (void)hipFree(c.d_ne_off);
+  (void)hipFree(c.d_diag_off);
   (void)hipFree(c.d_ne_list);
   (void)hipFree(c.d_colour_elems);
   free_chunk_plans(&c);

@@ -102,6 +102,7 @@ struct Ctx {
   long long *d_row_ptr = nullptr;
   int *d_col_idx = nullptr;
   uint8_t *d_slot = nullptr;  // (Ne*n*n) position of node b in node a's sorted neighbour list
+  int *d_diag_off = nullptr;  // (Ndof) offset of the diagonal entry within each CSR row
   int *d_ne_off = nullptr;    // (Nn+1) node -> incident (element, local index) entries
   int *d_ne_list = nullptr;   // (Ne*n) element * n + local index, element order within a node
   bool have_colour = false;

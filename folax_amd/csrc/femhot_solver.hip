@@ -189,6 +189,16 @@ __global__ void k_diag_inv(long long nrows, const long long *rp, const int *ci, 
   dinv[r] = (T)(dv != 0.0 ? 1.0 / dv : 1.0);
 }
 
+// The same on the context's own pattern, whose diagonal offsets are known: one value per row is read
+// instead of a binary search over the row's column indices.
+template <class T>
+__global__ void k_diag_inv_pos(long long nrows, const long long *rp, const int *doff, const double *v, T *dinv) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const double dv = doff[r] >= 0 ? v[rp[r] + doff[r]] : 0.0;
+  dinv[r] = (T)(dv != 0.0 ? 1.0 / dv : 1.0);
+}
+
 template <class T> __global__ void k_cast(long long n, const double *a, T *b) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long st = (long long)gridDim.x * blockDim.x;
@@ -458,7 +468,10 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
   rc = host_sum_partials(c, aux, s, &bb);
   if (rc) return rc;
   const double bnorm = std::sqrt(bb);
-  hipLaunchKernelGGL((k_diag_inv<double>), grid, dim3(256), 0, s, n, row_ptr, col_idx, vals, dinv64);
+  if (node_blocked)
+    hipLaunchKernelGGL((k_diag_inv_pos<double>), grid, dim3(256), 0, s, n, row_ptr, c->d_diag_off, vals, dinv64);
+  else
+    hipLaunchKernelGGL((k_diag_inv<double>), grid, dim3(256), 0, s, n, row_ptr, col_idx, vals, dinv64);
   int it = 0;
   double rel = 0.0;
   bool conv = false;
@@ -472,7 +485,10 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
     float *q32 = (float *)take(vec32), *dinv32 = (float *)take(vec32);
     const unsigned cg = (unsigned)std::min<long long>((nnz + 255) / 256, 65536);
     hipLaunchKernelGGL((k_cast<float>), dim3(cg), dim3(256), 0, s, nnz, vals, v32);
-    hipLaunchKernelGGL((k_diag_inv<float>), grid, dim3(256), 0, s, n, row_ptr, col_idx, vals, dinv32);
+    if (node_blocked)
+      hipLaunchKernelGGL((k_diag_inv_pos<float>), grid, dim3(256), 0, s, n, row_ptr, c->d_diag_off, vals, dinv32);
+    else
+      hipLaunchKernelGGL((k_diag_inv<float>), grid, dim3(256), 0, s, n, row_ptr, col_idx, vals, dinv32);
     // outer FP64 refinement: r = b - A x; solve A d = r in FP32 to inner_tol; x += d
     const double inner_tol = 1e-4;
     int outer = 0;
