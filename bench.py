@@ -236,7 +236,7 @@ def run_c3(args, torch, L):
 
         def step():
             ctx.residual(K, T, want_energy=False, flags=flags | L.MASK_DIRICHLET, out_R=R)
-            loss.copy_(torch.linalg.vector_norm(R, dtype=torch.float64).square())
+            loss.copy_(torch.linalg.vector_norm(R).square())  # the residual's own precision, one reduction
             ctx.residual_vjp(K, T, R, flags=flags | L.MASK_DIRICHLET, out=(gT, gk))
 
         for _ in range(args.warmup):

@@ -148,8 +148,12 @@ template <int ET, int NGP, int N, int P> struct RowsShape {
   static constexpr int OR = (P * P * KW + 63) / 64;                // output values per lane per window
 };
 
+// Waves per SIMD the compiler must fit: 3 for linear elasticity (168 VGPRs in FP64 HEX8 2x2x2, no spills);
+// the thermal instantiations keep the compiler's own choice (the powc of k(T) would spill under 3).
+template <int PH> constexpr int rows_min_waves() { return PH == MECH ? 3 : 1; }
+
 template <class R, int ET, int PH, int NGP>
-__global__ __launch_bounds__(64, 3) void k_tangent_rows(TArgs A) {
+__global__ __launch_bounds__(64, rows_min_waves<PH>()) void k_tangent_rows(TArgs A) {
   static_assert(PH != NEOHOOKE, "row-owner assembly covers thermal and linear elasticity");
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, P = E::P;
