@@ -37,7 +37,7 @@ EXPORTED = [
     "femhot_pattern", "femhot_colouring", "femhot_residual", "femhot_tangent_csr", "femhot_residual_vjp",
     "femhot_energy_grad", "femhot_pcg", "femhot_spmv", "femhot_orient_elements", "femhot_autotune",
     "femhot_kernel_timing", "femhot_set_chunk_config", "femhot_residual_shape_vjp", "femhot_newton",
-    "femhot_gauss_point_data", "femhot_energy_grad_noalloc",
+    "femhot_gauss_point_data", "femhot_energy_grad_noalloc", "femhot_pcg_async",
 ]
 
 OP_RESIDUAL, OP_VJP, OP_ENERGY_GRAD, OP_TANGENT = 0, 1, 2, 3
@@ -107,6 +107,7 @@ def lib():
     L.femhot_residual_shape_vjp.argtypes = [vp, i32, vp, vp, vp, vp, u32, vp]
     L.femhot_gauss_point_data.argtypes = [vp, i32, vp, vp, vp, vp]
     L.femhot_pcg.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, i32, C.POINTER(i32), C.POINTER(dbl), vp]
+    L.femhot_pcg_async.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, i32, vp, vp, vp, vp]
     L.femhot_spmv.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.femhot_newton.argtypes = [vp, i32, vp, vp, u32, i32, i32, dbl, dbl, dbl, i32, C.POINTER(dbl), C.POINTER(i32), vp]
     L.femhot_autotune.argtypes = [vp, i32, i32, i32, vp, ll, vp, ll, vp, ll, vp, ll, vp, ll, vp, u32,

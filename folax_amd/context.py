@@ -336,3 +336,22 @@ class FemhotContext:
             return x, it.value, rel.value, False
         L.check(rc)
         return x, it.value, rel.value, True
+
+    def pcg_async(self, row_ptr, col_idx, vals, b, x, tol=1e-10, iterations=1000, context_pattern=False,
+                  out=None):
+        """FP64 Jacobi-PCG with the convergence test on the device (femhot_pcg_async): x (initial guess,
+        solved in place), `iterations` launched, nothing waits for the host, so the call can be captured
+        into a graph.  Returns device tensors (iters int32, rel_res float64, status int32: 1 converged,
+        2 non-finite, 3 iterations spent); `out` may pass those three tensors in."""
+        if vals.dtype != torch.float64 or b.dtype != torch.float64 or x.dtype != torch.float64:
+            raise TypeError("pcg_async expects the FP64 tangent, right-hand side and solution")
+        if not x.is_contiguous() or x.numel() != self.n_dofs:
+            raise ValueError("pcg_async: x must be a contiguous vector of n_dofs")
+        if out is None:
+            out = (torch.empty(1, dtype=torch.int32, device=x.device), torch.empty(1, dtype=torch.float64, device=x.device),
+                   torch.empty(1, dtype=torch.int32, device=x.device))
+        it, rel, st = out
+        m = L.F64 | (L.PCG_CONTEXT_PATTERN if context_pattern else 0)
+        L.check(self._lib.femhot_pcg_async(self._h, m, _ptr(row_ptr), _ptr(col_idx), _ptr(vals), _ptr(b), _ptr(x),
+                                           float(tol), int(iterations), _ptr(it), _ptr(rel), _ptr(st), _stream()))
+        return it, rel, st

@@ -46,7 +46,9 @@ __device__ __forceinline__ double sum_partials(const double *part) {
 template <class T, int LPR>
 __global__ __launch_bounds__(256) void k_spmv(long long nrows, const long long *__restrict__ rp,
                                               const int *__restrict__ ci, const T *__restrict__ v,
-                                              const T *__restrict__ x, T *__restrict__ y, double *part) {
+                                              const T *__restrict__ x, T *__restrict__ y, double *part,
+                                              const int *stop) {
+  if (stop && *stop) return;  // a device-side solve that has stopped (uniform over the grid)
   constexpr int U = 8;
   const long long row = ((long long)blockIdx.x * 256 + threadIdx.x) / LPR;
   const int lane = (int)(threadIdx.x % LPR);
@@ -89,7 +91,9 @@ __global__ __launch_bounds__(256) void k_spmv(long long nrows, const long long *
 template <class T, int P, int LPN>
 __global__ __launch_bounds__(256) void k_spmv_node(long long nnodes, const long long *__restrict__ rp,
                                                    const int *__restrict__ ci, const T *__restrict__ v,
-                                                   const T *__restrict__ x, T *__restrict__ y, double *part) {
+                                                   const T *__restrict__ x, T *__restrict__ y, double *part,
+                                                   const int *stop) {
+  if (stop && *stop) return;
   constexpr int U = 4;
   const long long node = ((long long)blockIdx.x * 256 + threadIdx.x) / LPN;
   const int lane = (int)(threadIdx.x % LPN);
@@ -151,7 +155,9 @@ __global__ __launch_bounds__(256) void k_spmv_node(long long nnodes, const long 
 
 // out[0] = sum of part[0 .. n) in a fixed order (one workgroup of 1024; four independent
 // accumulators per thread keep four loads in flight).
-__global__ __launch_bounds__(1024) void k_sum_parts(const double *part, long long n, double *out) {
+__global__ __launch_bounds__(1024) void k_sum_parts(const double *part, long long n, double *out,
+                                                    const int *stop) {
+  if (stop && *stop) return;
   __shared__ double red[16];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   long long i = threadIdx.x;
@@ -229,7 +235,8 @@ __global__ __launch_bounds__(256) void k_init(long long n, const double *b, T *r
 template <class T>
 __global__ __launch_bounds__(256) void k_update_xr(long long n, T *x, T *r, const T *p, const T *q, const T *dinv,
                                                    const double *rz_part, const double *pap, double *rz_next,
-                                                   double *rr_next) {
+                                                   double *rr_next, const int *stop) {
+  if (stop && *stop) return;
   const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
   const long long j0 = i0 < n ? i0 : 0;  // n >= 1: index 0 is in bounds
   const T x0 = x[j0], p0 = p[j0], r0 = r[j0], q0 = q[j0], d0 = dinv[j0];
@@ -253,7 +260,8 @@ __global__ __launch_bounds__(256) void k_update_xr(long long n, T *x, T *r, cons
 // beta = (r.z)_new / (r.z)_old; p = Dinv r + beta p (first element loaded before the partial sums)
 template <class T>
 __global__ __launch_bounds__(256) void k_update_p(long long n, T *p, const T *r, const T *dinv, const double *rz_new,
-                                                  const double *rz_old) {
+                                                  const double *rz_old, const int *stop) {
+  if (stop && *stop) return;
   const long long i0 = (long long)blockIdx.x * 256 + threadIdx.x, st = (long long)gridDim.x * 256;
   const long long j0 = i0 < n ? i0 : 0;  // n >= 1: index 0 is in bounds
   const T r0 = r[j0], d0 = dinv[j0], p0 = p[j0];
@@ -302,28 +310,28 @@ static SpmvPlan spmv_plan(long long n, long long nnz, int P, bool node_blocked) 
 
 template <class T, int P>
 static void spmv_node_launch(long long nodes, const long long *rp, const int *ci, const T *v, const T *x, T *y,
-                             double *part, const SpmvPlan &sp, hipStream_t s) {
+                             double *part, const SpmvPlan &sp, hipStream_t s, const int *stop) {
   const dim3 grid(sp.blocks);
   switch (sp.lanes) {
-    case 2: hipLaunchKernelGGL((k_spmv_node<T, P, 2>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part); break;
-    case 4: hipLaunchKernelGGL((k_spmv_node<T, P, 4>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part); break;
-    case 8: hipLaunchKernelGGL((k_spmv_node<T, P, 8>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part); break;
-    default: hipLaunchKernelGGL((k_spmv_node<T, P, 16>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part); break;
+    case 2: hipLaunchKernelGGL((k_spmv_node<T, P, 2>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part, stop); break;
+    case 4: hipLaunchKernelGGL((k_spmv_node<T, P, 4>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part, stop); break;
+    case 8: hipLaunchKernelGGL((k_spmv_node<T, P, 8>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part, stop); break;
+    default: hipLaunchKernelGGL((k_spmv_node<T, P, 16>), grid, dim3(256), 0, s, nodes, rp, ci, v, x, y, part, stop); break;
   }
 }
 
 template <class T>
 static void spmv_launch(long long n, const long long *rp, const int *ci, const T *v, const T *x, T *y, double *part,
-                        const SpmvPlan &sp, hipStream_t s) {
-  if (sp.P == 3) return spmv_node_launch<T, 3>(n / 3, rp, ci, v, x, y, part, sp, s);
-  if (sp.P == 2) return spmv_node_launch<T, 2>(n / 2, rp, ci, v, x, y, part, sp, s);
+                        const SpmvPlan &sp, hipStream_t s, const int *stop = nullptr) {
+  if (sp.P == 3) return spmv_node_launch<T, 3>(n / 3, rp, ci, v, x, y, part, sp, s, stop);
+  if (sp.P == 2) return spmv_node_launch<T, 2>(n / 2, rp, ci, v, x, y, part, sp, s, stop);
   const dim3 grid(sp.blocks);
   switch (sp.lanes) {
-    case 4: hipLaunchKernelGGL((k_spmv<T, 4>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
-    case 8: hipLaunchKernelGGL((k_spmv<T, 8>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
-    case 16: hipLaunchKernelGGL((k_spmv<T, 16>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
-    case 32: hipLaunchKernelGGL((k_spmv<T, 32>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
-    default: hipLaunchKernelGGL((k_spmv<T, 64>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part); break;
+    case 4: hipLaunchKernelGGL((k_spmv<T, 4>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part, stop); break;
+    case 8: hipLaunchKernelGGL((k_spmv<T, 8>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part, stop); break;
+    case 16: hipLaunchKernelGGL((k_spmv<T, 16>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part, stop); break;
+    case 32: hipLaunchKernelGGL((k_spmv<T, 32>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part, stop); break;
+    default: hipLaunchKernelGGL((k_spmv<T, 64>), grid, dim3(256), 0, s, n, rp, ci, v, x, y, part, stop); break;
   }
 }
 
@@ -364,9 +372,11 @@ static int pcg_core(Ctx *c, long long n, const long long *rp, const int *ci, con
   while (k < maxiter) {
     const int par = k & 1;
     spmv_launch<T>(n, rp, ci, v, p, q, pap_part, sp, s);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, s, pap_part, (long long)nsp, pap);
-    hipLaunchKernelGGL((k_update_xr<T>), grid, dim3(256), 0, s, n, x, r, p, q, dinv, rz[par], pap, rz[1 - par], rrp);
-    hipLaunchKernelGGL((k_update_p<T>), grid, dim3(256), 0, s, n, p, r, dinv, rz[1 - par], rz[par]);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, s, pap_part, (long long)nsp, pap, (const int *)nullptr);
+    hipLaunchKernelGGL((k_update_xr<T>), grid, dim3(256), 0, s, n, x, r, p, q, dinv, rz[par], pap, rz[1 - par], rrp,
+                       (const int *)nullptr);
+    hipLaunchKernelGGL((k_update_p<T>), grid, dim3(256), 0, s, n, p, r, dinv, rz[1 - par], rz[par],
+                       (const int *)nullptr);
     ++k;
     if (k % check == 0 || k == maxiter) {
       rc = host_sum_partials(c, rrp, s, &rr);
@@ -386,6 +396,61 @@ static int pcg_core(Ctx *c, long long n, const long long *rp, const int *ci, con
   FH_HIP(hipGetLastError());
   *iters_out = k;
   return FEMHOT_OK;
+}
+
+// State of a device-side-stopped PCG (femhot_pcg_async): status 0 running, 1 converged, 2 non-finite
+// residual, 3 iteration budget spent.  Every guarded kernel returns at once when status != 0.
+struct PcgState {
+  int status;
+  int iters;
+  double bnorm2;
+  double rr;
+};
+
+// Sum (fixed order) of NBR partials by one workgroup of 1024 threads.
+__device__ __forceinline__ double sum_nbr_1024(const double *part) {
+  __shared__ double red[16];
+  double v = part[threadIdx.x];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int k = 0; k < 16; ++k) t += red[k];
+  return t;
+}
+
+// ||b||^2 and the initial ||r||^2 from their partials; converged at once when b = 0 or r is small enough
+__global__ __launch_bounds__(1024) void k_pcg_start(const double *bb_part, const double *rr_part, double tol2,
+                                                    int maxiter, PcgState *st) {
+  const double bb = sum_nbr_1024(bb_part);
+  const double rr = sum_nbr_1024(rr_part);
+  if (threadIdx.x == 0) {
+    st->bnorm2 = bb;
+    st->rr = rr;
+    st->iters = 0;
+    st->status = !(rr == rr && rr < INFINITY) ? 2 : (bb == 0.0 || rr <= tol2 * bb) ? 1 : (maxiter <= 0 ? 3 : 0);
+  }
+}
+
+// After one iteration's x, r update: count it, sum ||r||^2, stop on convergence, a non-finite residual or
+// the iteration budget
+__global__ __launch_bounds__(1024) void k_pcg_check(const double *rr_part, double tol2, int maxiter, PcgState *st) {
+  if (st->status) return;
+  const double rr = sum_nbr_1024(rr_part);
+  if (threadIdx.x == 0) {
+    const int it = st->iters + 1;
+    st->iters = it;
+    st->rr = rr;
+    st->status = !(rr == rr && rr < INFINITY) ? 2 : rr <= tol2 * st->bnorm2 ? 1 : (it >= maxiter ? 3 : 0);
+  }
+}
+
+__global__ void k_pcg_finish(const PcgState *st, int *iters, double *rel_res, int *status) {
+  if (threadIdx.x == 0) {
+    if (iters) *iters = st->iters;
+    if (rel_res) *rel_res = st->bnorm2 > 0.0 ? sqrt(st->rr / st->bnorm2) : 0.0;
+    if (status) *status = st->status;
+  }
 }
 
 // part[block] = this workgroup's share of a.b
@@ -529,6 +594,75 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
     set_error("femhot_pcg: maximum iterations reached");
     return FEMHOT_ERR_NOT_CONVERGED;
   }
+  return FEMHOT_OK;
+}
+
+extern "C" int femhot_pcg_async(femhot_ctx *h, int mode, const long long *row_ptr, const int *col_idx,
+                                const double *vals, const double *b, double *x, double tol, int iterations,
+                                int *iters, double *rel_res, int *status, void *stream) {
+  FH_CHECK_ARG(h && row_ptr && col_idx && vals && b && x, FEMHOT_ERR_ARG, "femhot_pcg_async: null argument");
+  const bool node_blocked = (mode & FEMHOT_PCG_CONTEXT_PATTERN) != 0;
+  mode &= ~FEMHOT_PCG_CONTEXT_PATTERN;
+  FH_CHECK_ARG(mode == FEMHOT_F64, FEMHOT_ERR_ARG, "femhot_pcg_async: mode must be FEMHOT_F64 (| FEMHOT_PCG_CONTEXT_PATTERN)");
+  FH_CHECK_ARG(tol > 0 && iterations > 0, FEMHOT_ERR_ARG, "femhot_pcg_async: tol and iterations must be positive");
+  Ctx *c = &h->c;
+  FH_CHECK_ARG(c->have_pattern, FEMHOT_ERR_ARG,
+               "femhot_pcg_async: build the context's pattern first (femhot_pattern or one tangent call)");
+  hipStream_t s = (hipStream_t)stream;
+  const long long n = c->Ndof, nnz = c->nnz;
+  const dim3 grid((unsigned)((n + 255) / 256)), nbr(NBR);
+  const size_t vec64 = sizeof(double) * n;
+  const SpmvPlan sp = spmv_plan(n, nnz, c->ndpn, node_blocked);
+  const size_t nsp = sp.blocks;
+  // the layout of femhot_pcg in FP64 mode plus the solver state, each piece 256-byte aligned
+  auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t need = r256(3 * NBR * sizeof(double)) + r256(NBR * sizeof(double)) + r256(nsp * sizeof(double)) +
+                      r256(8 * sizeof(double)) + 5 * r256(vec64) + r256(sizeof(PcgState));
+  if (c->ws_bytes < need) {  // sized once; never inside a graph capture
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FH_HIP(hipStreamIsCapturing(s, &cs));
+    FH_CHECK_ARG(cs == hipStreamCaptureStatusNone, FEMHOT_ERR_ARG,
+                 "femhot_pcg_async: the solver workspace is not sized; make one uncaptured call (or femhot_pcg) on "
+                 "this context first, so captured calls allocate nothing");
+    int rc = ensure_workspace(c, need);
+    if (rc) return rc;
+  }
+  char *ws = (char *)c->d_ws;
+  auto take = [&](size_t bytes) {
+    char *p = ws;
+    ws += (bytes + 255) & ~size_t(255);
+    return (void *)p;
+  };
+  double *sc = (double *)take(3 * NBR * sizeof(double));
+  double *aux = (double *)take(NBR * sizeof(double));
+  double *pap_part = (double *)take(nsp * sizeof(double));
+  double *pap = (double *)take(8 * sizeof(double));
+  double *r = (double *)take(vec64), *p = (double *)take(vec64), *q = (double *)take(vec64);
+  double *dinv = (double *)take(vec64);
+  (void)take(vec64);
+  PcgState *st = (PcgState *)take(sizeof(PcgState));
+  const int *stop = &st->status;
+  double *rz[2] = {sc, sc + NBR}, *rrp = sc + 2 * NBR;
+  const double tol2 = tol * tol;
+  hipLaunchKernelGGL(k_dot64, nbr, dim3(256), 0, s, n, b, b, aux);
+  if (node_blocked)
+    hipLaunchKernelGGL((k_diag_inv_pos<double>), grid, dim3(256), 0, s, n, row_ptr, c->d_diag_off, vals, dinv);
+  else
+    hipLaunchKernelGGL((k_diag_inv<double>), grid, dim3(256), 0, s, n, row_ptr, col_idx, vals, dinv);
+  spmv_launch<double>(n, row_ptr, col_idx, vals, x, r, nullptr, sp, s);  // r = A x
+  hipLaunchKernelGGL((k_init<double>), nbr, dim3(256), 0, s, n, b, r, p, dinv, rz[0], rrp);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1024), 0, s, aux, rrp, tol2, iterations, st);
+  for (int k = 0; k < iterations; ++k) {  // the same kernels as femhot_pcg; a stopped solve skips them
+    const int par = k & 1;
+    spmv_launch<double>(n, row_ptr, col_idx, vals, p, q, pap_part, sp, s, stop);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1024), 0, s, pap_part, (long long)nsp, pap, stop);
+    hipLaunchKernelGGL((k_update_xr<double>), nbr, dim3(256), 0, s, n, x, r, p, q, dinv, rz[par], pap, rz[1 - par],
+                       rrp, stop);
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(1024), 0, s, rrp, tol2, iterations, st);
+    hipLaunchKernelGGL((k_update_p<double>), nbr, dim3(256), 0, s, n, p, r, dinv, rz[1 - par], rz[par], stop);
+  }
+  hipLaunchKernelGGL(k_pcg_finish, dim3(1), dim3(64), 0, s, st, iters, rel_res, status);
+  FH_HIP(hipGetLastError());
   return FEMHOT_OK;
 }
 

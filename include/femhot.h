@@ -192,6 +192,19 @@ int femhot_pcg(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *c
                const double *b, double *x, double rel_tol, int maxiter, int *iters, double *rel_res,
                void *stream);
 
+/* femhot_pcg in FP64 with its convergence test on the device: the host enqueues exactly `iterations`
+ * iterations and never waits; once ||r|| <= rel_tol ||b|| (or the residual is not finite) the remaining
+ * iterations' kernels return at once.  iters (int), rel_res (double) and status (int: 1 converged,
+ * 2 non-finite residual, 3 iterations spent) are DEVICE pointers (any may be NULL), written at the end
+ * of the stream's work.  Allocates nothing once the context's solver workspace is sized (its first
+ * uncaptured call, or a femhot_pcg call), so a whole solve can be captured into a HIP graph.  mode:
+ * FEMHOT_F64 (| FEMHOT_PCG_CONTEXT_PATTERN); the context's pattern must be built.  No counterpart in the
+ * reference, whose iterative solve returns to the host (fe_solver.py:60-78); for callers that capture
+ * solves or keep the host out of the loop. */
+int femhot_pcg_async(femhot_ctx *ctx, int mode, const long long *row_ptr, const int *col_idx, const double *vals,
+                     const double *b, double *x, double rel_tol, int iterations, int *iters, double *rel_res,
+                     int *status, void *stream);
+
 /* Load-stepped Newton-Raphson solve of the BC-masked residual R(u; field) = 0, replacing
  * FiniteElementNonLinearResidualBasedSolver.Solve (fol/solvers/fe_nonlinear_residual_based_solver.py:
  * 76-170).  For load step s = 1..load_steps the Dirichlet dofs are set to (s / load_steps) x their
