@@ -37,7 +37,7 @@ EXPORTED = [
     "femhot_pattern", "femhot_colouring", "femhot_residual", "femhot_tangent_csr", "femhot_residual_vjp",
     "femhot_energy_grad", "femhot_pcg", "femhot_spmv", "femhot_orient_elements", "femhot_autotune",
     "femhot_kernel_timing", "femhot_set_chunk_config", "femhot_residual_shape_vjp", "femhot_newton",
-    "femhot_gauss_point_data", "femhot_energy_grad_noalloc", "femhot_pcg_async",
+    "femhot_gauss_point_data", "femhot_energy_grad_noalloc", "femhot_pcg_async", "femhot_newton_async",
 ]
 
 OP_RESIDUAL, OP_VJP, OP_ENERGY_GRAD, OP_TANGENT = 0, 1, 2, 3
@@ -110,6 +110,7 @@ def lib():
     L.femhot_pcg_async.argtypes = [vp, i32, vp, vp, vp, vp, vp, dbl, i32, vp, vp, vp, vp]
     L.femhot_spmv.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.femhot_newton.argtypes = [vp, i32, vp, vp, u32, i32, i32, dbl, dbl, dbl, i32, C.POINTER(dbl), C.POINTER(i32), vp]
+    L.femhot_newton_async.argtypes = [vp, vp, vp, u32, i32, i32, dbl, dbl, dbl, i32, vp, vp, vp, vp]
     L.femhot_autotune.argtypes = [vp, i32, i32, i32, vp, ll, vp, ll, vp, ll, vp, ll, vp, ll, vp, u32,
                                    C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_float), vp]
     L.femhot_orient_elements.argtypes = [i32, ll, C.POINTER(dbl), ll, C.POINTER(i32), C.POINTER(ll)]

@@ -320,6 +320,35 @@ class FemhotContext:
                                         _stream()))
         return x, np.array(hist[:], dtype=np.float64).reshape(load_steps, max_iter, 2), list(steps)
 
+    def newton_async(self, field, u, load_steps=1, max_iter=20, abs_tol=1e-8, rel_tol=1e-8, lin_tol=1e-10,
+                     lin_iterations=1000, flags=0, out=None):
+        """femhot_newton with every decision on the device (femhot_newton_async): u (FP64, initial guess)
+        is solved in place and nothing waits for the host, so the call can be captured into a graph
+        (after one uncaptured call on this context).  Returns device tensors: history (load_steps,
+        max_iter, 2) of (||R||, ||du||) with NaN after each step's last iteration, steps (load_steps)
+        int32, status (1) int32 (L.OK, or L.ERR_NONFINITE when a residual norm was NaN).  `out` may
+        pass those three tensors in."""
+        if u.dtype != torch.float64 or field.dtype != torch.float64:
+            raise TypeError("newton_async expects FP64 field and dofs")
+        if not u.is_contiguous() or u.numel() != self.n_dofs:
+            raise ValueError("newton_async: u must be a contiguous vector of n_dofs")
+        field = field.reshape(-1)
+        if not field.is_contiguous():
+            raise ValueError("newton_async: field must be contiguous")
+        self._check_in(u.view(1, -1), "u", self.n_dofs, 1, u.dtype, u.device)
+        self._check_in(field.view(1, -1), "field", self._field_size(flags), 1, u.dtype, u.device)
+        if out is None:
+            out = (torch.empty(load_steps, max_iter, 2, dtype=torch.float64, device=u.device),
+                   torch.empty(load_steps, dtype=torch.int32, device=u.device),
+                   torch.empty(1, dtype=torch.int32, device=u.device))
+        hist, steps, status = out
+        if hist.numel() != 2 * load_steps * max_iter or steps.numel() != load_steps:
+            raise ValueError("newton_async: out tensors do not match load_steps / max_iter")
+        L.check(self._lib.femhot_newton_async(self._h, _ptr(field), _ptr(u), int(flags), int(load_steps), int(max_iter),
+                                              float(abs_tol), float(rel_tol), float(lin_tol), int(lin_iterations),
+                                              _ptr(hist), _ptr(steps), _ptr(status), _stream()))
+        return hist, steps, status
+
     def pcg(self, row_ptr, col_idx, vals, b, x0=None, tol=1e-10, maxiter=10000, mode="f64", raise_on_fail=True,
             context_pattern=False):
         """Jacobi-PCG (FP64), or FP32 inner iterations + FP64 refinement (mode='mixed').  context_pattern:

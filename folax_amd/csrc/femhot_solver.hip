@@ -419,16 +419,21 @@ __device__ __forceinline__ double sum_nbr_1024(const double *part) {
   return t;
 }
 
-// ||b||^2 and the initial ||r||^2 from their partials; converged at once when b = 0 or r is small enough
+// ||b||^2 and the initial ||r||^2 from their partials; converged at once when b = 0 or r is small enough.
+// A nonzero *outer (the enclosing Newton solve has finished this load step, or failed) stops the solve
+// before its first iteration with status 4.
 __global__ __launch_bounds__(1024) void k_pcg_start(const double *bb_part, const double *rr_part, double tol2,
-                                                    int maxiter, PcgState *st) {
+                                                    int maxiter, const int *outer, PcgState *st) {
   const double bb = sum_nbr_1024(bb_part);
   const double rr = sum_nbr_1024(rr_part);
   if (threadIdx.x == 0) {
     st->bnorm2 = bb;
     st->rr = rr;
     st->iters = 0;
-    st->status = !(rr == rr && rr < INFINITY) ? 2 : (bb == 0.0 || rr <= tol2 * bb) ? 1 : (maxiter <= 0 ? 3 : 0);
+    st->status = (outer && *outer)              ? 4
+                 : !(rr == rr && rr < INFINITY) ? 2
+                 : (bb == 0.0 || rr <= tol2 * bb) ? 1
+                                                  : (maxiter <= 0 ? 3 : 0);
   }
 }
 
@@ -597,6 +602,21 @@ extern "C" int femhot_pcg(femhot_ctx *h, int mode, const long long *row_ptr, con
   return FEMHOT_OK;
 }
 
+// Workspace of femhot_pcg_async: the layout of femhot_pcg in FP64 mode plus the solver state, each piece
+// 256-byte aligned.
+static size_t pcg_async_ws_bytes(const Ctx *c, bool node_blocked) {
+  auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t vec64 = sizeof(double) * c->Ndof;
+  const size_t nsp = spmv_plan(c->Ndof, c->nnz, c->ndpn, node_blocked).blocks;
+  return r256(3 * NBR * sizeof(double)) + r256(NBR * sizeof(double)) + r256(nsp * sizeof(double)) +
+         r256(8 * sizeof(double)) + 5 * r256(vec64) + r256(sizeof(PcgState));
+}
+
+static int pcg_async_core(Ctx *c, bool node_blocked, const long long *row_ptr, const int *col_idx,
+                          const double *vals, const double *b, double *x, double tol, int iterations,
+                          const int *outer, const PcgState **state_out, int *iters, double *rel_res, int *status,
+                          hipStream_t s);
+
 extern "C" int femhot_pcg_async(femhot_ctx *h, int mode, const long long *row_ptr, const int *col_idx,
                                 const double *vals, const double *b, double *x, double tol, int iterations,
                                 int *iters, double *rel_res, int *status, void *stream) {
@@ -608,16 +628,23 @@ extern "C" int femhot_pcg_async(femhot_ctx *h, int mode, const long long *row_pt
   Ctx *c = &h->c;
   FH_CHECK_ARG(c->have_pattern, FEMHOT_ERR_ARG,
                "femhot_pcg_async: build the context's pattern first (femhot_pattern or one tangent call)");
-  hipStream_t s = (hipStream_t)stream;
+  return pcg_async_core(c, node_blocked, row_ptr, col_idx, vals, b, x, tol, iterations, nullptr, nullptr, iters,
+                        rel_res, status, (hipStream_t)stream);
+}
+
+// The enqueued solve of femhot_pcg_async.  outer: a device flag that, when nonzero as the solve starts,
+// skips it (status 4; the Newton driver's end-of-step flag); *state_out: where the solver state lives
+// until the next solve on this context.
+static int pcg_async_core(Ctx *c, bool node_blocked, const long long *row_ptr, const int *col_idx,
+                          const double *vals, const double *b, double *x, double tol, int iterations,
+                          const int *outer, const PcgState **state_out, int *iters, double *rel_res, int *status,
+                          hipStream_t s) {
   const long long n = c->Ndof, nnz = c->nnz;
   const dim3 grid((unsigned)((n + 255) / 256)), nbr(NBR);
   const size_t vec64 = sizeof(double) * n;
   const SpmvPlan sp = spmv_plan(n, nnz, c->ndpn, node_blocked);
   const size_t nsp = sp.blocks;
-  // the layout of femhot_pcg in FP64 mode plus the solver state, each piece 256-byte aligned
-  auto r256 = [](size_t b) { return (b + 255) & ~size_t(255); };
-  const size_t need = r256(3 * NBR * sizeof(double)) + r256(NBR * sizeof(double)) + r256(nsp * sizeof(double)) +
-                      r256(8 * sizeof(double)) + 5 * r256(vec64) + r256(sizeof(PcgState));
+  const size_t need = pcg_async_ws_bytes(c, node_blocked);
   if (c->ws_bytes < need) {  // sized once; never inside a graph capture
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     FH_HIP(hipStreamIsCapturing(s, &cs));
@@ -651,7 +678,7 @@ extern "C" int femhot_pcg_async(femhot_ctx *h, int mode, const long long *row_pt
     hipLaunchKernelGGL((k_diag_inv<double>), grid, dim3(256), 0, s, n, row_ptr, col_idx, vals, dinv);
   spmv_launch<double>(n, row_ptr, col_idx, vals, x, r, nullptr, sp, s);  // r = A x
   hipLaunchKernelGGL((k_init<double>), nbr, dim3(256), 0, s, n, b, r, p, dinv, rz[0], rrp);
-  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1024), 0, s, aux, rrp, tol2, iterations, st);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(1024), 0, s, aux, rrp, tol2, iterations, outer, st);
   for (int k = 0; k < iterations; ++k) {  // the same kernels as femhot_pcg; a stopped solve skips them
     const int par = k & 1;
     spmv_launch<double>(n, row_ptr, col_idx, vals, p, q, pap_part, sp, s, stop);
@@ -663,6 +690,7 @@ extern "C" int femhot_pcg_async(femhot_ctx *h, int mode, const long long *row_pt
   }
   hipLaunchKernelGGL(k_pcg_finish, dim3(1), dim3(64), 0, s, st, iters, rel_res, status);
   FH_HIP(hipGetLastError());
+  if (state_out) *state_out = st;
   return FEMHOT_OK;
 }
 
@@ -774,5 +802,179 @@ extern "C" int femhot_newton(femhot_ctx *h, int mode, const double *field, doubl
   }
   FH_HIP(hipGetLastError());
   FH_HIP(hipStreamSynchronize(s));
+  return FEMHOT_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// The same load-stepped Newton solve with every decision on the device: nothing waits for the host,
+// so a whole nonlinear solve can be captured into a HIP graph.
+// ------------------------------------------------------------------------------------------
+
+struct NewtonState {
+  int halt;    // this load step has ended (or the solve failed): the guarded kernels return
+  int failed;  // a NaN residual norm or a non-finite linear solve: every later kernel returns
+  double res_norm;
+};
+
+__global__ void k_newton_init(NewtonState *st, double *history, long long n_hist, int *steps, int load_steps) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    st->halt = 0;
+    st->failed = 0;
+    st->res_norm = 0.0;
+  }
+  if (history && i < n_hist) history[i] = __builtin_nan("");
+  if (steps && i < load_steps) steps[i] = 0;
+}
+__global__ void k_newton_begin_step(NewtonState *st) { st->halt = st->failed; }
+__global__ void k_apply_dirichlet_if(long long n, double *u, const uint8_t *dmask, const double *dval, double scale,
+                                     const int *failed) {
+  if (*failed) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && dmask[i]) u[i] = scale * dval[i];
+}
+__global__ void k_newton_rhs_if(long long n, const double *R, double *b, double *du, const int *halt) {
+  if (*halt) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    b[i] = -R[i];
+    du[i] = 0.0;
+  }
+}
+__global__ void k_add64_if(long long n, double *u, const double *du, const int *halt) {
+  if (*halt) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] += du[i];
+}
+// ||R|| of this iteration (from k_dot64's partials) into the state and the history; a NaN ends the solve
+__global__ __launch_bounds__(1024) void k_newton_res(const double *part, NewtonState *st, double *hist,
+                                                     int *step_iters, int it) {
+  if (st->halt) return;
+  const double rn = sqrt(sum_nbr_1024(part));
+  if (threadIdx.x == 0) {
+    st->res_norm = rn;
+    if (hist) hist[0] = rn;
+    if (rn != rn) {
+      st->failed = 1;
+      st->halt = 1;
+      if (step_iters) *step_iters = it;
+    }
+  }
+}
+// ||du|| after the linear solve; the load step ends when ||R|| < abs_tol, ||du|| < rel_tol or at max_iter
+// (the update of that last iteration is not applied), the solve when the linear solve was non-finite
+__global__ __launch_bounds__(1024) void k_newton_du(const double *part, NewtonState *st, const PcgState *ps,
+                                                    double *hist, int *step_iters, int it, int max_iter,
+                                                    double abs_tol, double rel_tol) {
+  if (st->halt) return;
+  const double dn = sqrt(sum_nbr_1024(part));
+  if (threadIdx.x == 0) {
+    if (ps->status == 2) {
+      st->failed = 1;
+      st->halt = 1;
+      if (step_iters) *step_iters = it;
+      return;
+    }
+    if (hist) hist[1] = dn;
+    if (st->res_norm < abs_tol || dn < rel_tol || it == max_iter) {
+      st->halt = 1;
+      if (step_iters) *step_iters = it;
+    }
+  }
+}
+__global__ void k_newton_finish(const NewtonState *st, int *status) {
+  if (threadIdx.x == 0 && status) *status = st->failed ? FEMHOT_ERR_NONFINITE : FEMHOT_OK;
+}
+
+extern "C" int femhot_newton_async(femhot_ctx *h, const double *field, double *u, unsigned flags, int load_steps,
+                                   int max_iter, double abs_tol, double rel_tol, double lin_tol, int lin_iterations,
+                                   double *history, int *steps, int *status, void *stream) {
+  FH_CHECK_ARG(h && field && u, FEMHOT_ERR_ARG, "femhot_newton_async: null argument");
+  FH_CHECK_ARG(load_steps > 0 && max_iter > 0 && lin_tol > 0 && lin_iterations > 0, FEMHOT_ERR_ARG,
+               "femhot_newton_async: load_steps, max_iter, lin_tol and lin_iterations must be positive");
+  FH_CHECK_ARG((flags & ~(FEMHOT_ELEMENT_FIELD | FEMHOT_SCATTER_COLOUR | FEMHOT_SCATTER_CHUNK | FEMHOT_SCATTER_ROWS)) ==
+                   0,
+               FEMHOT_ERR_ARG,
+               "femhot_newton_async: flags may hold FEMHOT_ELEMENT_FIELD and FEMHOT_SCATTER_{COLOUR,CHUNK,ROWS} only");
+  Ctx *c = &h->c;
+  hipStream_t s = (hipStream_t)stream;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  FH_HIP(hipStreamIsCapturing(s, &cs));
+  const bool capturing = cs != hipStreamCaptureStatusNone;
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const long long n = c->Ndof;
+  // the workspace of femhot_newton followed by the solve's state
+  const size_t need = up(sizeof(double) * std::max<long long>(c->nnz, 1)) + 4 * up(sizeof(double) * n) +
+                      up(sizeof(double) * NBR) + up(sizeof(NewtonState));
+  if (!c->have_pattern || c->newton_ws.bytes < need || !c->newton_dval_ready ||
+      c->ws_bytes < pcg_async_ws_bytes(c, true)) {
+    FH_CHECK_ARG(!capturing, FEMHOT_ERR_ARG,
+                 "femhot_newton_async: make one uncaptured call on this context first (it builds the pattern and sizes "
+                 "the workspaces), so captured calls allocate nothing");
+    int rc = ensure_pattern(c);
+    if (rc) return rc;
+    const size_t need_now = up(sizeof(double) * std::max<long long>(c->nnz, 1)) + 4 * up(sizeof(double) * n) +
+                            up(sizeof(double) * NBR) + up(sizeof(NewtonState));
+    if (c->newton_ws.bytes < need_now) {
+      FH_HIP(hipStreamSynchronize(s));
+      if (c->newton_ws.ptr) FH_HIP(hipFree(c->newton_ws.ptr));
+      c->newton_ws.ptr = nullptr;
+      c->newton_ws.bytes = 0;
+      c->newton_dval_ready = false;
+      FH_HIP(hipMalloc(&c->newton_ws.ptr, need_now));
+      c->newton_ws.bytes = need_now;
+    }
+    if ((rc = ensure_workspace(c, pcg_async_ws_bytes(c, true)))) return rc;
+  }
+  char *ws = (char *)c->newton_ws.ptr;
+  auto take = [&](size_t bytes) {
+    char *p = ws;
+    ws += up(bytes);
+    return (void *)p;
+  };
+  double *vals = (double *)take(sizeof(double) * std::max<long long>(c->nnz, 1));
+  double *R = (double *)take(sizeof(double) * n), *b = (double *)take(sizeof(double) * n);
+  double *du = (double *)take(sizeof(double) * n), *dval = (double *)take(sizeof(double) * n);
+  double *part = (double *)take(sizeof(double) * NBR);
+  NewtonState *st = (NewtonState *)take(sizeof(NewtonState));
+  if (!c->newton_dval_ready) {  // Dirichlet values as a dense dof vector (the last entry of a repeated dof wins)
+    std::vector<double> hv((size_t)n, 0.0);
+    for (size_t k = 0; k < c->h_dir_idx.size(); ++k) hv[(size_t)c->h_dir_idx[k]] = c->h_dir_val[k];
+    FH_HIP(hipStreamSynchronize(s));
+    FH_HIP(hipMemcpy(dval, hv.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    c->newton_dval_ready = true;
+  }
+  const long long n_hist = 2LL * load_steps * max_iter;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  const long long n_init = std::max<long long>({n_hist, (long long)load_steps, 1});
+  hipLaunchKernelGGL(k_newton_init, dim3((unsigned)((n_init + 255) / 256)), dim3(256), 0, s, st, history, n_hist,
+                     steps, load_steps);
+  const int *halt = &st->halt, *failed = &st->failed;
+  for (int step = 1; step <= load_steps; ++step) {
+    hipLaunchKernelGGL(k_apply_dirichlet_if, grid, dim3(256), 0, s, n, u, c->d_dmask, dval, (double)step / load_steps,
+                       failed);
+    hipLaunchKernelGGL(k_newton_begin_step, dim3(1), dim3(1), 0, s, st);
+    int *step_iters = steps ? steps + (step - 1) : nullptr;
+    for (int it = 1; it <= max_iter; ++it) {
+      double *hist = history ? history + 2LL * ((long long)(step - 1) * max_iter + (it - 1)) : nullptr;
+      // the tangent and residual are assembled whatever the flags say (their outputs are this call's
+      // workspace; u only changes through the guarded update below)
+      int rc = femhot_tangent_csr(h, FEMHOT_F64, field, u, vals, R, flags, stream);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_dot64, dim3(NBR), dim3(256), 0, s, n, R, R, part);
+      hipLaunchKernelGGL(k_newton_res, dim3(1), dim3(1024), 0, s, part, st, hist, step_iters, it);
+      hipLaunchKernelGGL(k_newton_rhs_if, grid, dim3(256), 0, s, n, R, b, du, halt);
+      const PcgState *ps = nullptr;
+      rc = pcg_async_core(c, true, c->d_row_ptr, c->d_col_idx, vals, b, du, lin_tol, lin_iterations, halt, &ps,
+                          nullptr, nullptr, nullptr, s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_dot64, dim3(NBR), dim3(256), 0, s, n, du, du, part);
+      hipLaunchKernelGGL(k_newton_du, dim3(1), dim3(1024), 0, s, part, st, ps, hist, step_iters, it, max_iter, abs_tol,
+                         rel_tol);
+      hipLaunchKernelGGL(k_add64_if, grid, dim3(256), 0, s, n, u, du, halt);
+    }
+  }
+  hipLaunchKernelGGL(k_newton_finish, dim3(1), dim3(64), 0, s, st, status);
+  FH_HIP(hipGetLastError());
   return FEMHOT_OK;
 }

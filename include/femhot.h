@@ -225,6 +225,25 @@ int femhot_newton(femhot_ctx *ctx, int mode, const double *field, double *u, uns
                   int max_iter, double abs_tol, double rel_tol, double lin_tol, int lin_maxiter, double *history,
                   int *steps, void *stream);
 
+/* femhot_newton with every decision taken on the device: the same load steps, assembly, norms and
+ * stopping rules (fe_nonlinear_residual_based_solver.py:76-170), with the FP64 linear solve of
+ * femhot_pcg_async (lin_iterations enqueued per Newton iteration; a solve that spends them continues
+ * with its last iterate).  The host enqueues load_steps x max_iter iterations and never waits; once a
+ * load step has ended its remaining iterations' updates and linear solves return at once (the tangent
+ * and residual are still assembled, into the context's workspace), and once the solve has failed
+ * every later kernel returns.  history: DEVICE (load_steps x max_iter x 2) doubles or NULL, (||R||,
+ * ||du||) per iteration, NaN after a load step's last iteration.  steps: DEVICE (load_steps) ints or
+ * NULL, iterations per load step (0 for steps after a failure).  status: DEVICE int or NULL,
+ * FEMHOT_OK, or FEMHOT_ERR_NONFINITE when a residual norm was NaN or a linear solve non-finite (the
+ * step's count is then the failing iteration).  The first call on a context must be uncaptured (it
+ * builds the pattern and sizes the workspaces); later calls allocate nothing, so a whole nonlinear
+ * solve can be captured into a HIP graph.  No counterpart in the reference, whose Newton loop
+ * returns to the host each iteration; for callers that capture solves or keep the host out of the
+ * loop. */
+int femhot_newton_async(femhot_ctx *ctx, const double *field, double *u, unsigned flags, int load_steps, int max_iter,
+                        double abs_tol, double rel_tol, double lin_tol, int lin_iterations, double *history,
+                        int *steps, int *status, void *stream);
+
 /* Autotuner of the scatter path that FEMHOT_SCATTER_CHUNK leaves to the library: times
  * elements-per-chunk x workgroup size x grid shape (one workgroup per chunk, or a persistent grid
  * of k workgroups per CU) x samples per work item of the LDS-staged chunk kernel on these inputs,
