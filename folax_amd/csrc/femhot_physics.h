@@ -383,9 +383,227 @@ __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)
 // ------------------------------------------------------------------------------------------
 // potential and its gradient
 // ------------------------------------------------------------------------------------------
+// ------------------------------------------------------------------------------------------
+// HEX8 in trilinear modes.  With the node signs of hexahedra_3d_8.py:79-88 (s_xi, s_eta, s_zeta of
+// node a), dN_a/dxi = 1/8 s_xi (1 + s_eta eta)(1 + s_zeta zeta), so sum_a v_a dN_a/dxi needs only the
+// eight moments m_w = sum_a w_a v_a of v over the products w of the node signs:
+//   8 dv/dxi   = m_xi   + m_xieta eta + m_xizeta zeta + m_xietazeta eta zeta   (and cyclically).
+// The moments of the coordinates and of the solution are formed once per element (24 adds per
+// component) instead of contracting 8 nodes at every Gauss point, the 1/8 cancels between J and the
+// solution gradient, and the nodal forces f (P inv(J)^T)_ik dN_a/dxi_k are accumulated as moments too
+// and spread to the nodes once at the end.  Mode index: 0 = 1, 1 = xi, 2 = eta, 3 = zeta, 4 = xi eta,
+// 5 = xi zeta, 6 = eta zeta, 7 = xi eta zeta.
+// ------------------------------------------------------------------------------------------
+template <class R> __device__ __forceinline__ void hex8_moments(const R (&v)[8], R (&m)[8]) {
+  // node a = (s_xi, s_eta, s_zeta): 0 (---), 1 (+--), 2 (++-), 3 (-+-), 4 (--+), 5 (+-+), 6 (+++), 7 (-++)
+  const R s00 = v[0] + v[1], d00 = v[1] - v[0], s10 = v[3] + v[2], d10 = v[2] - v[3];
+  const R s01 = v[4] + v[5], d01 = v[5] - v[4], s11 = v[7] + v[6], d11 = v[6] - v[7];
+  const R ss0 = s00 + s10, sd0 = s10 - s00, ss1 = s01 + s11, sd1 = s11 - s01;
+  const R ds0 = d00 + d10, dd0 = d10 - d00, ds1 = d01 + d11, dd1 = d11 - d01;
+  m[0] = ss0 + ss1;
+  m[3] = ss1 - ss0;
+  m[2] = sd0 + sd1;
+  m[6] = sd1 - sd0;
+  m[1] = ds0 + ds1;
+  m[5] = ds1 - ds0;
+  m[4] = dd0 + dd1;
+  m[7] = dd1 - dd0;
+}
+
+// v_a = sum_w w_a M_w (the transpose of hex8_moments)
+template <class R> __device__ __forceinline__ void hex8_spread(const R (&M)[8], R (&v)[8]) {
+  R t[2][4];  // [zeta sign][1, xi, eta, xi eta]
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const R sz = k ? R(1) : R(-1);
+    t[k][0] = M[0] + sz * M[3];
+    t[k][1] = M[1] + sz * M[5];
+    t[k][2] = M[2] + sz * M[6];
+    t[k][3] = M[4] + sz * M[7];
+  }
+  // (xi sign, eta sign) of nodes 0..3 (bottom) and 4..7 (top)
+  const int sx[4] = {-1, 1, 1, -1}, sy[4] = {-1, -1, 1, 1};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const R c0 = sy[a] > 0 ? t[k][0] + t[k][2] : t[k][0] - t[k][2];
+      const R c1 = sy[a] > 0 ? t[k][1] + t[k][3] : t[k][1] - t[k][3];
+      v[4 * k + a] = sx[a] > 0 ? c0 + c1 : c0 - c1;
+    }
+}
+
+// 8 d v/d(xi, eta, zeta) at (X, Y, Z) from the moments of v
+template <class R>
+__device__ __forceinline__ void hex8_dxi(const R (&m)[8], R X, R Y, R Z, R XY, R XZ, R YZ, R (&g)[3]) {
+  g[0] = ((m[1] + m[4] * Y) + m[5] * Z) + m[7] * YZ;
+  g[1] = ((m[2] + m[4] * X) + m[6] * Z) + m[7] * XZ;
+  g[2] = ((m[3] + m[5] * X) + m[6] * Y) + m[7] * XY;
+}
+
+// Unroll factor of the modal Gauss-point loop, by measurement on HEX8 48^3 x 16: as gp_unroll for
+// thermal and linear elasticity; two points at a time for FP32 Neo-Hookean (fully unrolled, the
+// chunk kernel needs 256 VGPRs and runs one wave per SIMD; 2 points: 217 VGPRs, two waves, 4 % faster
+// than the nodal form); rolled for FP64 Neo-Hookean.
+template <class R, int PH, int G> constexpr int hex8_unroll() {
+  return PH != NEOHOOKE ? gp_unroll<R, PH, G>() : (sizeof(R) == 4 ? 2 : 1);
+}
+
+// Potential and gradient of one HEX8 element through its modes (the same quantities as the generic
+// elem_egrad below, summed in another order).
+template <class R, int PH, int NGP>
+__device__ __forceinline__ void elem_egrad_hex8(const Elem<R, HEX8, PH> &el, const PhysParams &pp,
+                                                R (&r)[Elem<R, HEX8, PH>::NL], R (&gd)[8], R &Pi) {
+  using E = Elem<R, HEX8, PH>;
+  constexpr int N = 8, D = 3, P = E::P, G = RuleSize<HEX8, NGP>::G, UNR = hex8_unroll<R, PH, G>();
+  R mx[D][8], mu[P][8], M[P][8];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    R col[8];
+#pragma unroll
+    for (int a = 0; a < N; ++a) col[a] = el.x[a][i];
+    hex8_moments<R>(col, mx[i]);
+  }
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    R col[8];
+#pragma unroll
+    for (int a = 0; a < N; ++a) col[a] = el.u[a * P + c];
+    hex8_moments<R>(col, mu[c]);
+#pragma unroll
+    for (int w = 0; w < 8; ++w) M[c][w] = R(0);
+  }
+#pragma unroll
+  for (int a = 0; a < N; ++a) gd[a] = R(-0.0);
+  R pi = R(-0.0);
+#pragma unroll UNR
+  for (int g = 0; g < G; ++g) {
+    double xi, eta, zeta, w;
+    gauss_point<HEX8, NGP>(g, xi, eta, zeta, w);
+    R Nv[N], dNl[N][D];
+    shape<HEX8, R>(xi, eta, zeta, Nv, dNl);
+    const R X = R(xi), Y = R(eta), Z = R(zeta), XY = R(xi * eta), XZ = R(xi * zeta), YZ = R(eta * zeta);
+    // 8 J (J[i][k] = dx_i/dxi_k), its inverse and determinant
+    R J[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) hex8_dxi<R>(mx[i], X, Y, Z, XY, XZ, YZ, J[i]);
+    const R c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const R c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const R c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const R det8 = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const R id = recip(det8);
+    R inv[D][D];  // inverse of 8 J
+    inv[0][0] = c00 * id;
+    inv[1][0] = c01 * id;
+    inv[2][0] = c02 * id;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    const R wdet = R(w * (1.0 / 512.0)) * det8;
+    const R eg = field_gp<R, N>(el.d, el.de, el.elemental, Nv);
+    // solution gradient Gu[c][j] = (8 du_c/dxi) . inv(8 J)
+    R Gu[P][D];
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      R hx[D];
+      hex8_dxi<R>(mu[c], X, Y, Z, XY, XZ, YZ, hx);
+#pragma unroll
+      for (int j = 0; j < D; ++j) Gu[c][j] = (hx[0] * inv[0][j] + hx[1] * inv[1][j]) + hx[2] * inv[2][j];
+    }
+    // Q[c][k] = sum_j A[c][j] inv[k][j] for the flux / stress A of the physics (scaled by its factor)
+    R Q[P][D], q;
+    if constexpr (PH == THERMAL) {  // beta = 0 (checked on the host)
+      const R f = wdet * eg;
+#pragma unroll
+      for (int k = 0; k < D; ++k) Q[0][k] = f * ((inv[k][0] * Gu[0][0] + inv[k][1] * Gu[0][1]) + inv[k][2] * Gu[0][2]);
+      q = R(0.5) * wdet * ((Gu[0][0] * Gu[0][0] + Gu[0][1] * Gu[0][1]) + Gu[0][2] * Gu[0][2]);
+    } else if constexpr (PH == MECH) {
+      R S[D][D], Sf[D][D];
+      lin_stress<R, D>(Gu, pp, S);
+      scale_sym<R, D>(S, wdet * eg, Sf);
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) Q[i][k] = (Sf[i][0] * inv[k][0] + Sf[i][1] * inv[k][1]) + Sf[i][2] * inv[k][2];
+      q = R(0.5) * wdet * ddot_sym<R, D>(Gu, S);
+    } else {
+      NHState<R, D> st;
+      nh_state<R, D>(Gu, R(pp.kap1), R(pp.mu1), st);
+      const R f = wdet * eg;
+      R Pf[D][D];
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) Pf[i][j] = f * ((st.F[i][0] * st.S[0][j] + st.F[i][1] * st.S[1][j]) + st.F[i][2] * st.S[2][j]);
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int k = 0; k < D; ++k) Q[i][k] = (Pf[i][0] * inv[k][0] + Pf[i][1] * inv[k][1]) + Pf[i][2] * inv[k][2];
+      q = wdet * st.psi;
+    }
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      M[c][1] += Q[c][0];
+      M[c][2] += Q[c][1];
+      M[c][3] += Q[c][2];
+      M[c][4] += Q[c][0] * Y + Q[c][1] * X;
+      M[c][5] += Q[c][0] * Z + Q[c][2] * X;
+      M[c][6] += Q[c][1] * Z + Q[c][2] * Y;
+      M[c][7] += (Q[c][0] * YZ + Q[c][1] * XZ) + Q[c][2] * XY;
+    }
+    R work = R(-0.0);
+    if (pp.has_body) {  // body force b: r_a -= wdet N_a b, work = wdet b . u(x_g); N_a = 1/8 sum_w w_a w(x_g)
+      const R mode[8] = {R(1), X, Y, Z, XY, XZ, YZ, R(xi * eta * zeta)};
+#pragma unroll
+      for (int c = 0; c < P; ++c) {
+        const R b = R(PH == THERMAL ? pp.body[0] : pp.body[c]);
+        const R fb = wdet * b * R(0.125);
+        R uc = R(-0.0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          M[c][k] -= fb * mode[k];
+          uc += mu[c][k] * mode[k];
+        }
+        work += fb * uc;
+      }
+    }
+    pi += eg * q - work;
+    if (el.elemental)
+      gd[0] += q;
+    else {
+#pragma unroll
+      for (int a = 0; a < N; ++a) gd[a] += q * Nv[a];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    R v[8];
+    hex8_spread<R>(M[c], v);
+#pragma unroll
+    for (int a = 0; a < N; ++a) r[a * P + c] = v[a];
+  }
+  Pi = pi;
+}
+
+template <class R, int ET, int PH, int NGP>
+__device__ __forceinline__ void elem_egrad_generic(const Elem<R, ET, PH> &el, const PhysParams &pp,
+                                                   R (&r)[Elem<R, ET, PH>::NL], R (&gd)[Elem<R, ET, PH>::N], R &Pi);
+
 template <class R, int ET, int PH, int NGP>
 __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const PhysParams &pp,
                                            R (&r)[Elem<R, ET, PH>::NL], R (&gd)[Elem<R, ET, PH>::N], R &Pi) {
+  if constexpr (ET == HEX8)
+    elem_egrad_hex8<R, PH, NGP>(el, pp, r, gd, Pi);
+  else
+    elem_egrad_generic<R, ET, PH, NGP>(el, pp, r, gd, Pi);
+}
+
+template <class R, int ET, int PH, int NGP>
+__device__ __forceinline__ void elem_egrad_generic(const Elem<R, ET, PH> &el, const PhysParams &pp,
+                                                   R (&r)[Elem<R, ET, PH>::NL], R (&gd)[Elem<R, ET, PH>::N], R &Pi) {
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, G = RuleSize<ET, NGP>::G, UNR = gp_unroll<R, PH, G>();
 #pragma unroll
