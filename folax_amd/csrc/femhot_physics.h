@@ -196,9 +196,26 @@ __device__ __forceinline__ void nh_grad_u(const R (&u)[N * D], const R (&DN)[N][
 // ------------------------------------------------------------------------------------------
 // residual + reference energy
 // ------------------------------------------------------------------------------------------
+template <class R, int PH, int NGP, bool RES>
+__device__ __forceinline__ void elem_hex8_modal(const Elem<R, HEX8, PH> &el, const PhysParams &pp,
+                                                R (&r)[Elem<R, HEX8, PH>::NL], R *gd, R &Pi);
+
+template <class R, int ET, int PH, int NGP>
+__device__ __forceinline__ void elem_residual_nodal(const Elem<R, ET, PH> &el, const PhysParams &pp,
+                                                    R (&r)[Elem<R, ET, PH>::NL], R &energy);
+
 template <class R, int ET, int PH, int NGP>
 __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const PhysParams &pp,
                                               R (&r)[Elem<R, ET, PH>::NL], R &energy) {
+  if constexpr (ET == HEX8)
+    elem_hex8_modal<R, PH, NGP, true>(el, pp, r, nullptr, energy);
+  else
+    elem_residual_nodal<R, ET, PH, NGP>(el, pp, r, energy);
+}
+
+template <class R, int ET, int PH, int NGP>
+__device__ __forceinline__ void elem_residual_nodal(const Elem<R, ET, PH> &el, const PhysParams &pp,
+                                                    R (&r)[Elem<R, ET, PH>::NL], R &energy) {
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, G = RuleSize<ET, NGP>::G, UNR = gp_unroll<R, PH, G>();
 #pragma unroll
@@ -449,11 +466,12 @@ template <class R, int PH, int G> constexpr int hex8_unroll() {
   return PH != NEOHOOKE ? gp_unroll<R, PH, G>() : (sizeof(R) == 4 ? 2 : 1);
 }
 
-// Potential and gradient of one HEX8 element through its modes (the same quantities as the generic
-// elem_egrad below, summed in another order).
-template <class R, int PH, int NGP>
-__device__ __forceinline__ void elem_egrad_hex8(const Elem<R, HEX8, PH> &el, const PhysParams &pp,
-                                                R (&r)[Elem<R, HEX8, PH>::NL], R (&gd)[8], R &Pi) {
+// One HEX8 element through its modes: with RES the residual and reference energy of elem_residual
+// (the thermal conductivity k0 (1 + beta T^c) included; gd unused), else the potential and gradient of
+// elem_egrad (gd = dPi/dd) -- the same quantities as the nodal forms below, summed in another order.
+template <class R, int PH, int NGP, bool RES>
+__device__ __forceinline__ void elem_hex8_modal(const Elem<R, HEX8, PH> &el, const PhysParams &pp,
+                                                R (&r)[Elem<R, HEX8, PH>::NL], R *gd, R &Pi) {
   using E = Elem<R, HEX8, PH>;
   constexpr int N = 8, D = 3, P = E::P, G = RuleSize<HEX8, NGP>::G, UNR = hex8_unroll<R, PH, G>();
   R mx[D][8], mu[P][8], M[P][8];
@@ -473,8 +491,10 @@ __device__ __forceinline__ void elem_egrad_hex8(const Elem<R, HEX8, PH> &el, con
 #pragma unroll
     for (int w = 0; w < 8; ++w) M[c][w] = R(0);
   }
+  if constexpr (!RES) {
 #pragma unroll
-  for (int a = 0; a < N; ++a) gd[a] = R(-0.0);
+    for (int a = 0; a < N; ++a) gd[a] = R(-0.0);
+  }
   R pi = R(-0.0);
 #pragma unroll UNR
   for (int g = 0; g < G; ++g) {
@@ -483,6 +503,7 @@ __device__ __forceinline__ void elem_egrad_hex8(const Elem<R, HEX8, PH> &el, con
     R Nv[N], dNl[N][D];
     shape<HEX8, R>(xi, eta, zeta, Nv, dNl);
     const R X = R(xi), Y = R(eta), Z = R(zeta), XY = R(xi * eta), XZ = R(xi * zeta), YZ = R(eta * zeta);
+    const R mode[8] = {R(1), X, Y, Z, XY, XZ, YZ, R(xi * eta * zeta)};  // N_a = 1/8 sum_w w_a mode[w]
     // 8 J (J[i][k] = dx_i/dxi_k), its inverse and determinant
     R J[D][D];
 #pragma unroll
@@ -515,8 +536,15 @@ __device__ __forceinline__ void elem_egrad_hex8(const Elem<R, HEX8, PH> &el, con
     }
     // Q[c][k] = sum_j A[c][j] inv[k][j] for the flux / stress A of the physics (scaled by its factor)
     R Q[P][D], q;
-    if constexpr (PH == THERMAL) {  // beta = 0 (checked on the host)
-      const R f = wdet * eg;
+    if constexpr (PH == THERMAL) {  // egrad: beta = 0 (checked on the host)
+      R k = eg;
+      if constexpr (RES) {
+        R Tg = R(-0.0);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) Tg += mu[0][m] * mode[m];
+        k = eg * (R(1) + R(pp.beta) * powc(Tg * R(0.125), R(pp.c)));
+      }
+      const R f = wdet * k;
 #pragma unroll
       for (int k = 0; k < D; ++k) Q[0][k] = f * ((inv[k][0] * Gu[0][0] + inv[k][1] * Gu[0][1]) + inv[k][2] * Gu[0][2]);
       q = R(0.5) * wdet * ((Gu[0][0] * Gu[0][0] + Gu[0][1] * Gu[0][1]) + Gu[0][2] * Gu[0][2]);
@@ -555,8 +583,7 @@ __device__ __forceinline__ void elem_egrad_hex8(const Elem<R, HEX8, PH> &el, con
       M[c][7] += (Q[c][0] * YZ + Q[c][1] * XZ) + Q[c][2] * XY;
     }
     R work = R(-0.0);
-    if (pp.has_body) {  // body force b: r_a -= wdet N_a b, work = wdet b . u(x_g); N_a = 1/8 sum_w w_a w(x_g)
-      const R mode[8] = {R(1), X, Y, Z, XY, XZ, YZ, R(xi * eta * zeta)};
+    if (pp.has_body) {  // body force b: r_a -= wdet N_a b, work = wdet b . u(x_g)
 #pragma unroll
       for (int c = 0; c < P; ++c) {
         const R b = R(PH == THERMAL ? pp.body[0] : pp.body[c]);
@@ -570,13 +597,23 @@ __device__ __forceinline__ void elem_egrad_hex8(const Elem<R, HEX8, PH> &el, con
         work += fb * uc;
       }
     }
-    pi += eg * q - work;
-    if (el.elemental)
-      gd[0] += q;
-    else {
+    if constexpr (RES) {
+      if constexpr (PH == NEOHOOKE) pi += eg * q;  // the reference energy: sum of wdet e psi
+    } else {
+      pi += eg * q - work;
+      if (el.elemental)
+        gd[0] += q;
+      else {
 #pragma unroll
-      for (int a = 0; a < N; ++a) gd[a] += q * Nv[a];
+        for (int a = 0; a < N; ++a) gd[a] += q * Nv[a];
+      }
     }
+  }
+  if constexpr (RES && PH != NEOHOOKE) {  // the reference energy u . r = sum_w (moment w of u) M_w
+#pragma unroll
+    for (int c = 0; c < P; ++c)
+#pragma unroll
+      for (int m = 0; m < 8; ++m) pi += mu[c][m] * M[c][m];
   }
 #pragma unroll
   for (int c = 0; c < P; ++c) {
@@ -596,7 +633,7 @@ template <class R, int ET, int PH, int NGP>
 __device__ __forceinline__ void elem_egrad(const Elem<R, ET, PH> &el, const PhysParams &pp,
                                            R (&r)[Elem<R, ET, PH>::NL], R (&gd)[Elem<R, ET, PH>::N], R &Pi) {
   if constexpr (ET == HEX8)
-    elem_egrad_hex8<R, PH, NGP>(el, pp, r, gd, Pi);
+    elem_hex8_modal<R, PH, NGP, false>(el, pp, r, gd, Pi);
   else
     elem_egrad_generic<R, ET, PH, NGP>(el, pp, r, gd, Pi);
 }
