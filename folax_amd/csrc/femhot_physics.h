@@ -194,6 +194,164 @@ __device__ __forceinline__ void nh_grad_u(const R (&u)[N * D], const R (&DN)[N][
 }
 
 // ------------------------------------------------------------------------------------------
+// QUAD4 thermal in bilinear modes.  With the node signs of quadrilateral_2d_4.py:17-66 (s_xi, s_eta of
+// node a), sum_a v_a dN_a/dxi = 1/4 (m_xi + m_xieta eta) and sum_a v_a dN_a/deta = 1/4 (m_eta + m_xieta
+// xi) for the moments m_w = sum_a w_a v_a over the sign products (0 = 1, 1 = xi, 2 = eta, 3 = xi eta).
+// The moments of the coordinates, the temperature and the cotangent are formed once per element; the
+// 1/4 cancels between J and the gradients, and the nodal terms accumulate as moments spread to the
+// nodes once (as for HEX8 below).
+// ------------------------------------------------------------------------------------------
+template <class R> __device__ __forceinline__ void quad4_moments(const R (&v)[4], R (&m)[4]) {
+  // node a = (s_xi, s_eta): 0 (--), 1 (+-), 2 (++), 3 (-+)
+  const R s0 = v[0] + v[1], d0 = v[1] - v[0], s1 = v[3] + v[2], d1 = v[2] - v[3];
+  m[0] = s0 + s1;
+  m[2] = s1 - s0;
+  m[1] = d0 + d1;
+  m[3] = d1 - d0;
+}
+
+// v_a = M_0 + s_xi M_1 + s_eta M_2 + s_xi s_eta M_3
+template <class R> __device__ __forceinline__ void quad4_spread(const R (&M)[4], R (&v)[4]) {
+  const R lo0 = M[0] - M[2], lo1 = M[1] - M[3], hi0 = M[0] + M[2], hi1 = M[1] + M[3];
+  v[0] = lo0 - lo1;
+  v[1] = lo0 + lo1;
+  v[2] = hi0 + hi1;
+  v[3] = hi0 - hi1;
+}
+
+// Gauss-point geometry of a QUAD4 from its coordinate moments: inv(4 J) and w det J.
+template <class R>
+__device__ __forceinline__ void quad4_modal_geometry(const R (&mx)[2][4], R X, R Y, double w, R (&inv)[2][2],
+                                                     R &wdet) {
+  const R j00 = mx[0][1] + mx[0][3] * Y, j01 = mx[0][2] + mx[0][3] * X;
+  const R j10 = mx[1][1] + mx[1][3] * Y, j11 = mx[1][2] + mx[1][3] * X;
+  const R det4 = j00 * j11 - j01 * j10;
+  const R id = recip(det4);
+  inv[0][0] = j11 * id;
+  inv[0][1] = -j01 * id;
+  inv[1][0] = -j10 * id;
+  inv[1][1] = j00 * id;
+  wdet = R(w * (1.0 / 16.0)) * det4;
+}
+
+// grad v at the Gauss point from the moments of v: (4 dv/dxi) . inv(4 J)
+template <class R>
+__device__ __forceinline__ void quad4_modal_grad(const R (&m)[4], R X, R Y, const R (&inv)[2][2], R (&g)[2]) {
+  const R gx = m[1] + m[3] * Y, gy = m[2] + m[3] * X;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) g[j] = gx * inv[0][j] + gy * inv[1][j];
+}
+
+// M += the moments of f dN_a/dx . q (the flux term of the nodal vector): Q = f inv(4 J) q
+template <class R>
+__device__ __forceinline__ void quad4_modal_flux(R f, const R (&q)[2], const R (&inv)[2][2], R X, R Y, R (&M)[4]) {
+  const R Q0 = f * (inv[0][0] * q[0] + inv[0][1] * q[1]);
+  const R Q1 = f * (inv[1][0] * q[0] + inv[1][1] * q[1]);
+  M[1] += Q0;
+  M[2] += Q1;
+  M[3] += Q0 * Y + Q1 * X;
+}
+
+// Residual and reference energy T . r of a QUAD4 thermal element (elem_residual's quantities).
+template <class R, int NGP>
+__device__ __forceinline__ void elem_residual_quad4_thermal(const Elem<R, QUAD4, THERMAL> &el, const PhysParams &pp,
+                                                            R (&r)[4], R &energy) {
+  constexpr int N = 4, G = RuleSize<QUAD4, NGP>::G, UNR = gp_unroll<R, THERMAL, G>();
+  R mx[2][4], mu[4], M[4] = {R(0), R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const R col[4] = {el.x[0][i], el.x[1][i], el.x[2][i], el.x[3][i]};
+    quad4_moments<R>(col, mx[i]);
+  }
+  quad4_moments<R>(el.u, mu);
+#pragma unroll UNR
+  for (int g = 0; g < G; ++g) {
+    double xi, eta, zeta, w;
+    gauss_point<QUAD4, NGP>(g, xi, eta, zeta, w);
+    R Nv[N], dNl[N][2];
+    shape<QUAD4, R>(xi, eta, zeta, Nv, dNl);
+    const R X = R(xi), Y = R(eta);
+    const R mode[4] = {R(1), X, Y, R(xi * eta)};  // N_a = 1/4 sum_w w_a mode[w]
+    R inv[2][2], wdet;
+    quad4_modal_geometry<R>(mx, X, Y, w, inv, wdet);
+    const R eg = field_gp<R, N>(el.d, el.de, el.elemental, Nv);
+    R Tg = R(-0.0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) Tg += mu[m] * mode[m];
+    Tg = Tg * R(0.25);
+    R gT[2];
+    quad4_modal_grad<R>(mu, X, Y, inv, gT);
+    const R k = eg * (R(1) + R(pp.beta) * powc(Tg, R(pp.c)));
+    quad4_modal_flux<R>(wdet * k, gT, inv, X, Y, M);
+    if (pp.has_body) {
+      const R fb = wdet * R(pp.body[0]) * R(0.25);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) M[m] -= fb * mode[m];
+    }
+  }
+  R en = R(-0.0);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) en += mu[m] * M[m];
+  quad4_spread<R>(M, r);
+  energy = en;
+}
+
+// VJP of a QUAD4 thermal element (elem_vjp's quantities): gu = (dr/dT)^T w (Picard: the k(T) term
+// dropped), gd = (dr/dk0)^T w.
+template <class R, int NGP>
+__device__ __forceinline__ void elem_vjp_quad4_thermal(const Elem<R, QUAD4, THERMAL> &el, const R (&wv)[4],
+                                                       const PhysParams &pp, bool picard, R (&gu)[4], R (&gd)[4]) {
+  constexpr int N = 4, G = RuleSize<QUAD4, NGP>::G, UNR = gp_unroll<R, THERMAL, G>();
+  R mx[2][4], mu[4], mw[4], M[4] = {R(0), R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const R col[4] = {el.x[0][i], el.x[1][i], el.x[2][i], el.x[3][i]};
+    quad4_moments<R>(col, mx[i]);
+  }
+  quad4_moments<R>(el.u, mu);
+  quad4_moments<R>(wv, mw);
+#pragma unroll
+  for (int a = 0; a < N; ++a) gd[a] = R(-0.0);
+  const R bt = R(pp.beta);
+#pragma unroll UNR
+  for (int g = 0; g < G; ++g) {
+    double xi, eta, zeta, w;
+    gauss_point<QUAD4, NGP>(g, xi, eta, zeta, w);
+    R Nv[N], dNl[N][2];
+    shape<QUAD4, R>(xi, eta, zeta, Nv, dNl);
+    const R X = R(xi), Y = R(eta);
+    const R mode[4] = {R(1), X, Y, R(xi * eta)};
+    R inv[2][2], wdet;
+    quad4_modal_geometry<R>(mx, X, Y, w, inv, wdet);
+    const R eg = field_gp<R, N>(el.d, el.de, el.elemental, Nv);
+    R Tg = R(-0.0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) Tg += mu[m] * mode[m];
+    Tg = Tg * R(0.25);
+    R gT[2], gW[2];
+    quad4_modal_grad<R>(mu, X, Y, inv, gT);
+    quad4_modal_grad<R>(mw, X, Y, inv, gW);
+    const R wGT = wdet * (gW[0] * gT[0] + gW[1] * gT[1]);  // w^T G_g T
+    const R fac = R(1) + bt * powc(Tg, R(pp.c));
+    quad4_modal_flux<R>(wdet * eg * fac, gW, inv, X, Y, M);
+    if (!picard && bt != R(0)) {  // d k / d T: gu_a += wGT dk N_a
+      const R dk = eg * bt * dpowc(Tg, R(pp.c));
+      const R t = wGT * dk * R(0.25);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) M[m] += t * mode[m];
+    }
+    const R q = wGT * fac;
+    if (el.elemental)
+      gd[0] += q;
+    else {
+#pragma unroll
+      for (int a = 0; a < N; ++a) gd[a] += q * Nv[a];
+    }
+  }
+  quad4_spread<R>(M, gu);
+}
+
+// ------------------------------------------------------------------------------------------
 // residual + reference energy
 // ------------------------------------------------------------------------------------------
 template <class R, int PH, int NGP, bool RES>
@@ -209,6 +367,8 @@ __device__ __forceinline__ void elem_residual(const Elem<R, ET, PH> &el, const P
                                               R (&r)[Elem<R, ET, PH>::NL], R &energy) {
   if constexpr (ET == HEX8)
     elem_hex8_modal<R, PH, NGP, true>(el, pp, r, nullptr, energy);
+  else if constexpr (ET == QUAD4 && PH == THERMAL)
+    elem_residual_quad4_thermal<R, NGP>(el, pp, r, energy);
   else
     elem_residual_nodal<R, ET, PH, NGP>(el, pp, r, energy);
 }
@@ -299,6 +459,10 @@ template <class R, int ET, int PH, int NGP>
 __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)[Elem<R, ET, PH>::NL],
                                          const PhysParams &pp, bool picard, R (&gu)[Elem<R, ET, PH>::NL],
                                          R (&gd)[Elem<R, ET, PH>::N]) {
+  if constexpr (ET == QUAD4 && PH == THERMAL) {  // bilinear modes (above)
+    elem_vjp_quad4_thermal<R, NGP>(el, w, pp, picard, gu, gd);
+    return;
+  }
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, G = RuleSize<ET, NGP>::G, UNR = gp_unroll<R, PH, G>();
 #pragma unroll
