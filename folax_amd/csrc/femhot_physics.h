@@ -455,12 +455,20 @@ __device__ __forceinline__ void elem_residual_nodal(const Elem<R, ET, PH> &el, c
 // ------------------------------------------------------------------------------------------
 // residual VJP
 // ------------------------------------------------------------------------------------------
+template <class R, int PH, int NGP>
+__device__ __forceinline__ void elem_vjp_hex8(const Elem<R, HEX8, PH> &el, const R (&wv)[Elem<R, HEX8, PH>::NL],
+                                              const PhysParams &pp, bool picard, R (&gu)[Elem<R, HEX8, PH>::NL],
+                                              R (&gd)[8]);
+
 template <class R, int ET, int PH, int NGP>
 __device__ __forceinline__ void elem_vjp(const Elem<R, ET, PH> &el, const R (&w)[Elem<R, ET, PH>::NL],
                                          const PhysParams &pp, bool picard, R (&gu)[Elem<R, ET, PH>::NL],
                                          R (&gd)[Elem<R, ET, PH>::N]) {
   if constexpr (ET == QUAD4 && PH == THERMAL) {  // bilinear modes (above)
     elem_vjp_quad4_thermal<R, NGP>(el, w, pp, picard, gu, gd);
+    return;
+  } else if constexpr (ET == HEX8) {  // trilinear modes (below)
+    elem_vjp_hex8<R, PH, NGP>(el, w, pp, picard, gu, gd);
     return;
   }
   using E = Elem<R, ET, PH>;
@@ -787,6 +795,158 @@ __device__ __forceinline__ void elem_hex8_modal(const Elem<R, HEX8, PH> &el, con
     for (int a = 0; a < N; ++a) r[a * P + c] = v[a];
   }
   Pi = pi;
+}
+
+// VJP of one HEX8 element through its modes (elem_vjp's quantities, summed in another order): the
+// moments of the cotangent give its gradient as those of the solution give H, and every term of the
+// form A_ij dN_a/dx_j accumulates as moments spread to the nodes once.
+template <class R, int PH, int NGP>
+__device__ __forceinline__ void elem_vjp_hex8(const Elem<R, HEX8, PH> &el, const R (&wv)[Elem<R, HEX8, PH>::NL],
+                                              const PhysParams &pp, bool picard, R (&gu)[Elem<R, HEX8, PH>::NL],
+                                              R (&gd)[8]) {
+  using E = Elem<R, HEX8, PH>;
+  constexpr int N = 8, D = 3, P = E::P, G = RuleSize<HEX8, NGP>::G, UNR = hex8_unroll<R, PH, G>();
+  R mx[D][8], mu[P][8], mw[P][8], M[P][8];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    R col[8];
+#pragma unroll
+    for (int a = 0; a < N; ++a) col[a] = el.x[a][i];
+    hex8_moments<R>(col, mx[i]);
+  }
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    R cu[8], cw[8];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      cu[a] = el.u[a * P + c];
+      cw[a] = wv[a * P + c];
+    }
+    hex8_moments<R>(cu, mu[c]);
+    hex8_moments<R>(cw, mw[c]);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) M[c][m] = R(0);
+  }
+#pragma unroll
+  for (int a = 0; a < N; ++a) gd[a] = R(-0.0);
+#pragma unroll UNR
+  for (int g = 0; g < G; ++g) {
+    double xi, eta, zeta, w;
+    gauss_point<HEX8, NGP>(g, xi, eta, zeta, w);
+    R Nv[N], dNl[N][D];
+    shape<HEX8, R>(xi, eta, zeta, Nv, dNl);
+    const R X = R(xi), Y = R(eta), Z = R(zeta), XY = R(xi * eta), XZ = R(xi * zeta), YZ = R(eta * zeta);
+    const R mode[8] = {R(1), X, Y, Z, XY, XZ, YZ, R(xi * eta * zeta)};
+    R J[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) hex8_dxi<R>(mx[i], X, Y, Z, XY, XZ, YZ, J[i]);
+    const R c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const R c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const R c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const R det8 = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const R id = recip(det8);
+    R inv[D][D];  // inverse of 8 J
+    inv[0][0] = c00 * id;
+    inv[1][0] = c01 * id;
+    inv[2][0] = c02 * id;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    const R wdet = R(w * (1.0 / 512.0)) * det8;
+    const R eg = field_gp<R, N>(el.d, el.de, el.elemental, Nv);
+    R Gu[P][D], Gw[P][D];
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      R hu[D], hw[D];
+      hex8_dxi<R>(mu[c], X, Y, Z, XY, XZ, YZ, hu);
+      hex8_dxi<R>(mw[c], X, Y, Z, XY, XZ, YZ, hw);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        Gu[c][j] = (hu[0] * inv[0][j] + hu[1] * inv[1][j]) + hu[2] * inv[2][j];
+        Gw[c][j] = (hw[0] * inv[0][j] + hw[1] * inv[1][j]) + hw[2] * inv[2][j];
+      }
+    }
+    R A[P][D], q;  // gu_(a,c) += sum_j A[c][j] dN_a/dx_j
+    if constexpr (PH == THERMAL) {
+      R Tg = R(-0.0);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) Tg += mu[0][m] * mode[m];
+      Tg = Tg * R(0.125);
+      const R wGT = wdet * ((Gw[0][0] * Gu[0][0] + Gw[0][1] * Gu[0][1]) + Gw[0][2] * Gu[0][2]);
+      const R bt = R(pp.beta);
+      const R fac = R(1) + bt * powc(Tg, R(pp.c));
+      const R f = wdet * eg * fac;
+#pragma unroll
+      for (int j = 0; j < D; ++j) A[0][j] = f * Gw[0][j];
+      if (!picard && bt != R(0)) {  // d k / d T: gu_a += wGT dk N_a
+        const R t = wGT * (eg * bt * dpowc(Tg, R(pp.c))) * R(0.125);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) M[0][m] += t * mode[m];
+      }
+      q = wGT * fac;
+    } else if constexpr (PH == MECH) {
+      R Su[D][D], Sw[D][D];
+      lin_stress<R, D>(Gu, pp, Su);
+      lin_stress<R, D>(Gw, pp, Sw);
+      scale_sym<R, D>(Sw, wdet * eg, A);
+      q = wdet * ddot_sym<R, D>(Gw, Su);
+    } else {
+      NHState<R, D> st;
+      nh_state<R, D>(Gu, R(pp.kap1), R(pp.mu1), st);
+      R Xs[D][D];  // sym(F^T dF)
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          R s2 = R(-0.0);
+#pragma unroll
+          for (int k = 0; k < D; ++k) s2 += st.F[k][i] * Gw[k][j] + st.F[k][j] * Gw[k][i];
+          Xs[i][j] = R(0.5) * s2;
+        }
+      R dS[D][D];
+      nh_tangent_apply<R, D>(st, R(pp.kap1), R(pp.mu1), Xs, dS);
+      const R f = wdet * eg;
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          R s2 = R(-0.0);
+#pragma unroll
+          for (int k = 0; k < D; ++k) s2 += st.F[i][k] * dS[k][j] + Gw[i][k] * st.S[k][j];
+          A[i][j] = f * s2;
+        }
+      q = wdet * ddot_sym<R, D>(Xs, st.S);
+    }
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      R Q[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) Q[k] = (A[c][0] * inv[k][0] + A[c][1] * inv[k][1]) + A[c][2] * inv[k][2];
+      M[c][1] += Q[0];
+      M[c][2] += Q[1];
+      M[c][3] += Q[2];
+      M[c][4] += Q[0] * Y + Q[1] * X;
+      M[c][5] += Q[0] * Z + Q[2] * X;
+      M[c][6] += Q[1] * Z + Q[2] * Y;
+      M[c][7] += (Q[0] * YZ + Q[1] * XZ) + Q[2] * XY;
+    }
+    if (el.elemental)
+      gd[0] += q;
+    else {
+#pragma unroll
+      for (int a = 0; a < N; ++a) gd[a] += q * Nv[a];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    R v[8];
+    hex8_spread<R>(M[c], v);
+#pragma unroll
+    for (int a = 0; a < N; ++a) gu[a * P + c] = v[a];
+  }
 }
 
 template <class R, int ET, int PH, int NGP>
