@@ -1065,11 +1065,168 @@ __device__ __forceinline__ void elem_egrad_generic(const Elem<R, ET, PH> &el, co
 // ------------------------------------------------------------------------------------------
 // tangent rows of local node a: K[c][b*P+d] = Ke[(a,c),(b,d)]
 // ------------------------------------------------------------------------------------------
+// Row (a, c) of a HEX8 element tangent through the trilinear modes (the quantities of the nodal form
+// below): node a's gradient is formed from its signs, the solution gradient from the moments of u,
+// and every term A_dj dN_b/dx_j of the row accumulates as moments spread to the 8 columns once.
+template <class R, int PH, int NGP>
+__device__ __forceinline__ void elem_tangent_row_hex8(const Elem<R, HEX8, PH> &el, const PhysParams &pp, int a, int c,
+                                                      R (&K)[Elem<R, HEX8, PH>::NL]) {
+  using E = Elem<R, HEX8, PH>;
+  constexpr int N = 8, D = 3, P = E::P, G = RuleSize<HEX8, NGP>::G;
+  R mx[D][8], mu[P][8], M[P][8];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    R col[8];
+#pragma unroll
+    for (int b = 0; b < N; ++b) col[b] = el.x[b][i];
+    hex8_moments<R>(col, mx[i]);
+  }
+#pragma unroll
+  for (int d = 0; d < P; ++d) {
+    if constexpr (PH != MECH) {
+      R col[8];
+#pragma unroll
+      for (int b = 0; b < N; ++b) col[b] = el.u[b * P + d];
+      hex8_moments<R>(col, mu[d]);
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) M[d][m] = R(0);
+  }
+  // node a's signs (node order of hexahedra_3d_8.py:79-88)
+  const R sa0 = ((a & 3) == 1 || (a & 3) == 2) ? R(1) : R(-1);
+  const R sa1 = (a & 3) >= 2 ? R(1) : R(-1);
+  const R sa2 = a >= 4 ? R(1) : R(-1);
+#pragma unroll 1
+  for (int g = 0; g < G; ++g) {
+    double xi, eta, zeta, w;
+    gauss_point<HEX8, NGP>(g, xi, eta, zeta, w);
+    R Nv[N], dNl[N][D];
+    shape<HEX8, R>(xi, eta, zeta, Nv, dNl);
+    const R X = R(xi), Y = R(eta), Z = R(zeta), XY = R(xi * eta), XZ = R(xi * zeta), YZ = R(eta * zeta);
+    R J[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) hex8_dxi<R>(mx[i], X, Y, Z, XY, XZ, YZ, J[i]);
+    const R c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const R c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const R c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const R det8 = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const R id = recip(det8);
+    R inv[D][D];  // inverse of 8 J
+    inv[0][0] = c00 * id;
+    inv[1][0] = c01 * id;
+    inv[2][0] = c02 * id;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    const R wdet = R(w * (1.0 / 512.0)) * det8;
+    const R eg = field_gp<R, N>(el.d, el.de, el.elemental, Nv);
+    // dN_a/dx = (8 dN_a/dxi) . inv(8 J)
+    const R px = R(1) + sa0 * X, py = R(1) + sa1 * Y, pz = R(1) + sa2 * Z;
+    const R sh[3] = {sa0 * py * pz, sa1 * px * pz, sa2 * px * py};
+    R dNa[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) dNa[j] = (sh[0] * inv[0][j] + sh[1] * inv[1][j]) + sh[2] * inv[2][j];
+    R A[P][D];  // K_(b,d) += sum_j A[d][j] dN_b/dx_j
+    if constexpr (PH == THERMAL) {
+      R Tg = R(-0.0);
+      const R mode[8] = {R(1), X, Y, Z, XY, XZ, YZ, R(xi * eta * zeta)};
+#pragma unroll
+      for (int m = 0; m < 8; ++m) Tg += mu[0][m] * mode[m];
+      const R f = wdet * eg * (R(1) + R(pp.beta) * powc(Tg * R(0.125), R(pp.c)));
+#pragma unroll
+      for (int j = 0; j < D; ++j) A[0][j] = f * dNa[j];
+    } else if constexpr (PH == MECH) {
+      R Gc[D][D], S[D][D];
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) Gc[i][j] = (i == c) ? dNa[j] : R(0);
+      lin_stress<R, D>(Gc, pp, S);
+      const R f = wdet * eg;
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) A[i][j] = f * S[i][j];
+    } else {
+      R H[D][D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        R hu[D];
+        hex8_dxi<R>(mu[i], X, Y, Z, XY, XZ, YZ, hu);
+#pragma unroll
+        for (int j = 0; j < D; ++j) H[i][j] = (hu[0] * inv[0][j] + hu[1] * inv[1][j]) + hu[2] * inv[2][j];
+      }
+      NHState<R, D> st;
+      nh_state<R, D>(H, R(pp.kap1), R(pp.mu1), st);
+      R SdNa[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        R s2 = R(-0.0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) s2 += st.S[i][k] * dNa[k];
+        SdNa[i] = s2;
+      }
+      R Fc[D];
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        R s2 = st.F[0][i];
+#pragma unroll
+        for (int k = 1; k < D; ++k) s2 = (k == c) ? st.F[k][i] : s2;
+        Fc[i] = s2;
+      }
+      R Xs[D][D];
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) Xs[i][j] = R(0.5) * (Fc[i] * dNa[j] + Fc[j] * dNa[i]);
+      R dS[D][D];
+      nh_tangent_apply<R, D>(st, R(pp.kap1), R(pp.mu1), Xs, dS);
+      const R f = wdet * eg;
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          R s2 = (i == c) ? SdNa[j] : R(-0.0);
+#pragma unroll
+          for (int k = 0; k < D; ++k) s2 += st.F[i][k] * dS[k][j];
+          A[i][j] = f * s2;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < P; ++d) {
+      R Q[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) Q[k] = (A[d][0] * inv[k][0] + A[d][1] * inv[k][1]) + A[d][2] * inv[k][2];
+      M[d][1] += Q[0];
+      M[d][2] += Q[1];
+      M[d][3] += Q[2];
+      M[d][4] += Q[0] * Y + Q[1] * X;
+      M[d][5] += Q[0] * Z + Q[2] * X;
+      M[d][6] += Q[1] * Z + Q[2] * Y;
+      M[d][7] += (Q[0] * YZ + Q[1] * XZ) + Q[2] * XY;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < P; ++d) {
+    R v[8];
+    hex8_spread<R>(M[d], v);
+#pragma unroll
+    for (int b = 0; b < N; ++b) K[b * P + d] = v[b];
+  }
+}
+
 template <class R, int ET, int PH, int NGP>
 __device__ __forceinline__ void elem_tangent_row(const Elem<R, ET, PH> &el, const PhysParams &pp, int a, int c,
                                                  R (&K)[Elem<R, ET, PH>::NL]) {
   // Row (a, c) of the element tangent.  The Gauss loop is rolled and only one row is accumulated, so
   // the FP64 HEX8 case stays within the register file (24 accumulators instead of 72).
+  if constexpr (ET == HEX8) {
+    elem_tangent_row_hex8<R, PH, NGP>(el, pp, a, c, K);
+    return;
+  }
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, G = RuleSize<ET, NGP>::G;
 #pragma unroll
