@@ -53,12 +53,47 @@ __device__ __forceinline__ void shape_node(int a, double x, double y, double z, 
 // Inverse Jacobian of Gauss point g of one element for the row-owner staging, with the shape function
 // values there: the arithmetic of gp_geometry (coordinates relative to node 0, J = sum_a x_a (x)
 // dN_a/dxi, inverse by cofactors), returning wdetJ.  xs: the element's node coordinates (N x 3, FP64).
+// HEX8 takes the trilinear modes of femhot_physics.h instead: J from the moments of the coordinates
+// (adds only, then 4 terms per entry), and the pair (8 J)^-1 with stage_gradients' 8 dN/dxi.
 template <class R, int ET, int NGP>
 __device__ __forceinline__ R gp_inverse(int g, const double *xs, R (&Nv)[ElemT<ET>::N],
                                         R (&inv)[ElemT<ET>::D][ElemT<ET>::D]) {
   constexpr int N = ElemT<ET>::N, D = ElemT<ET>::D;
   double xi, eta, zeta, w;
   gauss_point<ET, NGP>(g, xi, eta, zeta, w);
+  if constexpr (ET == HEX8) {
+    R J[3][3];  // 8 J
+    const R X = R(xi), Y = R(eta), Z = R(zeta), XY = R(xi * eta), XZ = R(xi * zeta), YZ = R(eta * zeta);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      R col[8], m[8];
+      col[0] = R(0);
+#pragma unroll
+      for (int a = 1; a < 8; ++a) col[a] = R(xs[a * 3 + i] - xs[i]);
+      hex8_moments<R>(col, m);
+      hex8_dxi<R>(m, X, Y, Z, XY, XZ, YZ, J[i]);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      R dn[3];
+      shape_node<ET, R>(a, xi, eta, zeta, Nv[a], dn);
+    }
+    const R c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const R c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const R c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const R det8 = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const R id = recip(det8);
+    inv[0][0] = c00 * id;
+    inv[1][0] = c01 * id;
+    inv[2][0] = c02 * id;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    return R(w * (1.0 / 512.0)) * det8;
+  }
   double x0[D];
 #pragma unroll
   for (int i = 0; i < D; ++i) x0[i] = xs[i];
@@ -111,11 +146,25 @@ __device__ __forceinline__ R gp_inverse(int g, const double *xs, R (&Nv)[ElemT<E
 }
 
 // dN_b/dx = dN_b/dxi . inv(J) of every node at Gauss point g, stored at st[b * D + j] (TET4: the
-// rows of inv(J), node 0 minus their sum, as gp_geometry).
+// rows of inv(J), node 0 minus their sum, as gp_geometry; HEX8: (8 dN_b/dxi) . inv(8 J), from the
+// node signs).
 template <class R, int ET, int NGP>
 __device__ __forceinline__ void stage_gradients(int g, const R (&inv)[ElemT<ET>::D][ElemT<ET>::D], R *st) {
   constexpr int N = ElemT<ET>::N, D = ElemT<ET>::D;
-  if constexpr (ET == TET4) {
+  if constexpr (ET == HEX8) {
+    double xi, eta, zeta, w;
+    gauss_point<ET, NGP>(g, xi, eta, zeta, w);
+    const R X = R(xi), Y = R(eta), Z = R(zeta);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int q = b & 3;
+      const R sx = (q == 1 || q == 2) ? R(1) : R(-1), sy = q >= 2 ? R(1) : R(-1), sz = b >= 4 ? R(1) : R(-1);
+      const R px = R(1) + sx * X, py = R(1) + sy * Y, pz = R(1) + sz * Z;
+      const R sh[3] = {sx * (py * pz), sy * (px * pz), sz * (px * py)};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) st[b * 3 + j] = (sh[0] * inv[0][j] + sh[1] * inv[1][j]) + sh[2] * inv[2][j];
+    }
+  } else if constexpr (ET == TET4) {
 #pragma unroll
     for (int j = 0; j < D; ++j) {
       st[1 * D + j] = inv[0][j];
