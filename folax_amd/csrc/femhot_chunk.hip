@@ -43,6 +43,8 @@ template <class R> struct ReduceArgs {
   unsigned flags;
 };
 
+constexpr int REDUCE_NPT = 2;  // shared nodes per thread of k_reduce
+
 template <class R, int P>
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> A) {
   const bool accumulate = (A.flags & FEMHOT_ACCUMULATE) != 0;
@@ -63,46 +65,75 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> A) {
     }
     return;
   }
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= A.n_shared * A.batch) return;
-  const long long b = i / A.n_shared;
-  const long long k = i - b * A.n_shared;
-  const int g = A.sh_nodes[k];
-  const int j0 = A.red_off[k], j1 = A.red_off[k + 1];  // a shared node has a slot in >= 2 chunks
+  // REDUCE_NPT nodes per thread (node i and i + 256 of the block's range): the index and partial loads of
+  // both are issued before either sum, and the launch needs half as many threads
+  const long long tot = A.n_shared * A.batch;
   const bool mask = (A.flags & FEMHOT_MASK_DIRICHLET) != 0;
+  bool ok[REDUCE_NPT];
+  long long bu[REDUCE_NPT];
+  int gu[REDUCE_NPT], j0u[REDUCE_NPT], nu[REDUCE_NPT];
+#pragma unroll
+  for (int u = 0; u < REDUCE_NPT; ++u) {
+    const long long i = (long long)blockIdx.x * (256 * REDUCE_NPT) + u * 256 + threadIdx.x;
+    ok[u] = i < tot;
+    const long long ic = ok[u] ? i : tot - 1;  // clamped: the loads stay in range, the stores are skipped
+    bu[u] = ic / A.n_shared;
+    const long long k = ic - bu[u] * A.n_shared;
+    gu[u] = A.sh_nodes[k];
+    j0u[u] = A.red_off[k];
+    nu[u] = A.red_off[k + 1] - j0u[u];  // a shared node has a slot in >= 2 chunks
+  }
   // Every load a node needs is issued before the first add: its first two slots (consecutive
   // memory), the Dirichlet mask and, when accumulating, the old outputs.  Further slots (nodes in
   // 3+ chunks) follow in a loop.  The sum runs over the slots in chunk order.
   if (A.ou) {
-    const R *pu = A.part_u + b * A.n_slots * P + (long long)j0 * P;
-    R *o = A.ou + b * A.ous + (long long)g * P;
-    R v0[P], v1[P], old[P];
-    bool masked[P];
+    R v0[REDUCE_NPT][P], v1[REDUCE_NPT][P], old[REDUCE_NPT][P];
+    bool masked[REDUCE_NPT][P];
 #pragma unroll
-    for (int c = 0; c < P; ++c) {
-      v0[c] = pu[c];
-      v1[c] = j1 - j0 >= 2 ? pu[P + c] : R(-0.0);
-      masked[c] = mask && A.dmask[(long long)g * P + c];
-      old[c] = accumulate ? o[c] : R(0);
+    for (int u = 0; u < REDUCE_NPT; ++u) {
+      const R *pu = A.part_u + bu[u] * A.n_slots * P + (long long)j0u[u] * P;
+      const R *o = A.ou + bu[u] * A.ous + (long long)gu[u] * P;
+#pragma unroll
+      for (int c = 0; c < P; ++c) {
+        v0[u][c] = pu[c];
+        v1[u][c] = nu[u] >= 2 ? pu[P + c] : R(-0.0);
+        masked[u][c] = mask && A.dmask[(long long)gu[u] * P + c];
+        old[u][c] = accumulate ? o[c] : R(0);
+      }
     }
-    R s[P];
 #pragma unroll
-    for (int c = 0; c < P; ++c) s[c] = (R(-0.0) + v0[c]) + v1[c];
-    for (int j = 2; j < j1 - j0; ++j)
+    for (int u = 0; u < REDUCE_NPT; ++u) {
+      const R *pu = A.part_u + bu[u] * A.n_slots * P + (long long)j0u[u] * P;
+      R *o = A.ou + bu[u] * A.ous + (long long)gu[u] * P;
+      R s[P];
 #pragma unroll
-      for (int c = 0; c < P; ++c) s[c] += pu[j * P + c];
+      for (int c = 0; c < P; ++c) s[c] = (R(-0.0) + v0[u][c]) + v1[u][c];
+      for (int j = 2; j < nu[u]; ++j)
 #pragma unroll
-    for (int c = 0; c < P; ++c)
-      if (!(masked[c] && accumulate)) o[c] = masked[c] ? R(0) : (accumulate ? old[c] + s[c] : s[c]);
+        for (int c = 0; c < P; ++c) s[c] += pu[j * P + c];
+      if (ok[u]) {
+#pragma unroll
+        for (int c = 0; c < P; ++c)
+          if (!(masked[u][c] && accumulate)) o[c] = masked[u][c] ? R(0) : (accumulate ? old[u][c] + s[c] : s[c]);
+      }
+    }
   }
   if (A.od) {
-    const R *pd = A.part_d + b * A.n_slots + j0;
-    R *o = A.od + b * A.ods + g;
-    const R v0 = pd[0], v1 = j1 - j0 >= 2 ? pd[1] : R(-0.0);
-    const R old = accumulate ? *o : R(0);
-    R s = (R(-0.0) + v0) + v1;
-    for (int j = 2; j < j1 - j0; ++j) s += pd[j];
-    *o = accumulate ? old + s : s;
+    R v0[REDUCE_NPT], v1[REDUCE_NPT], old[REDUCE_NPT];
+#pragma unroll
+    for (int u = 0; u < REDUCE_NPT; ++u) {
+      const R *pd = A.part_d + bu[u] * A.n_slots + j0u[u];
+      v0[u] = pd[0];
+      v1[u] = nu[u] >= 2 ? pd[1] : R(-0.0);
+      old[u] = accumulate ? A.od[bu[u] * A.ods + gu[u]] : R(0);
+    }
+#pragma unroll
+    for (int u = 0; u < REDUCE_NPT; ++u) {
+      const R *pd = A.part_d + bu[u] * A.n_slots + j0u[u];
+      R s = (R(-0.0) + v0[u]) + v1[u];
+      for (int j = 2; j < nu[u]; ++j) s += pd[j];
+      if (ok[u]) A.od[bu[u] * A.ods + gu[u]] = accumulate ? old[u] + s : s;
+    }
   }
 }
 
@@ -302,7 +333,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   {
     const bool nodes_part = p.n_shared && (ou || want_gd_nodal);
     const long long tot = nodes_part ? p.n_shared * batch : 0;
-    const int nb_nodes = (int)((tot + 255) / 256);
+    const int nb_nodes = (int)((tot + 256 * REDUCE_NPT - 1) / (256 * REDUCE_NPT));
     const int nb_energy = energy ? c->nseg * batch : 0;
     if (nb_nodes + nb_energy > 0) {
       const long long os = batch > 1 ? ous : c->Ndof, dstr = batch > 1 ? ods : c->Nn;
