@@ -323,8 +323,8 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
         }
       }
       if (A.chunk_energy) {
-        en = wave_sum(en);
-        if ((tid & 63) == 0) red[tid >> 6] = en;
+        en = wave_sum_to_last(en);
+        if ((tid & 63) == 63) red[tid >> 6] = en;
       }
       __syncthreads();
       if (A.chunk_energy && tid == 0) {

@@ -303,8 +303,8 @@ __global__ __launch_bounds__(BLOCK) void k_chunk_pipe(ChunkArgs A) {
       }
     }
     if (A.chunk_energy) {
-      en = wave_sum(en);
-      if (lane == 0) red[wv] = en;
+      en = wave_sum_to_last(en);
+      if (lane == 63) red[wv] = en;
     }
     wait_all_memory();  // the next plan has landed (this wave's share)
     __syncthreads();    // ... every wave's share; the data buffer is free

@@ -47,6 +47,27 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// v of the lane DPP control CTRL names, rows outside ROWS (a 4-bit row mask) reading 0.
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave, valid in lane 63 only: pairs and quads (quad_perm), half rows and
+// rows (row_half_mirror, row_mirror), then row 0 into row 1 and row 2 into row 3 (row_bcast:15), then
+// rows 0-1 into rows 2-3 (row_bcast:31).  Register moves instead of the LDS round trips of a shuffle
+// butterfly; a fixed order, so the sum is the same on every run.
+__device__ __forceinline__ double wave_sum_to_last(double v) {
+  v += dpp_f64<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_f64<0x140, 0xf>(v);  // row_mirror
+  v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 // Sum `v` into energy[idx] with one atomic per wave when all active lanes share idx.
 __device__ __forceinline__ void add_energy(double *energy, int idx, double v, bool active) {
   const unsigned long long act = __ballot(active);
