@@ -103,12 +103,17 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL;
   // nodes / elements per lane per round of loads: enough that one round covers 512 nodes and
   // 1024 elements (a 1024-element chunk's nodes and elements) without idle registers at 512 threads
+  // (TET4 at 256 threads: 512 elements per round, as below)
   constexpr int KN = BLOCK >= 512 ? 1 : (BLOCK == 256 ? 2 : 4);
-  constexpr int KE = (ET != HEX8 && NGP <= 2) ? (BLOCK >= 512 ? 2 : 4) : 1;
   // Light element routines (several elements per lane per round) keep the first round of node
-  // metadata and element connectivity in registers from the plan load on; heavy ones (HEX8,
-  // high-order rules) need those registers for the element arithmetic and reload them where used.
-  constexpr bool PF = KE > 1 && BLOCK >= 512;
+  // metadata, element connectivity and per-element field in registers from the plan load on; heavy
+  // ones (HEX8, high-order rules) need those registers for the element arithmetic and reload them
+  // where used.  At 256 threads TET4 takes 2 elements per lane and the prefetch (119 VGPRs, still 4
+  // waves per SIMD: 2.5 us faster on the uneven TET4 batch), QUAD4 4 and none (the prefetch measured
+  // 1 us slower on the QUAD4 thermal batch).
+  constexpr bool PF256 = ET == TET4 && BLOCK == 256;
+  constexpr int KE = (ET != HEX8 && NGP <= 2) ? ((BLOCK >= 512 || PF256) ? 2 : 4) : 1;
+  constexpr bool PF = KE > 1 && (BLOCK >= 512 || PF256);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[BLOCK / 64];
   const bool elemental = (A.flags & FEMHOT_ELEMENT_FIELD) != 0;
