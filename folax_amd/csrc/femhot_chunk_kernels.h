@@ -376,11 +376,13 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
                 for (int c = 0; c < P; ++c) s[c] += e[c];
                 q = qn;
               }
+              if (sl[k] >= 0) {  // a shared node: its partial slot, the P values stored back to back
+                R *pu = PU + (long long)sl[k] * P;
 #pragma unroll
-              for (int c = 0; c < P; ++c) {
-                if (sl[k] >= 0) {
-                  PU[(long long)sl[k] * P + c] = s[c];
-                } else {
+                for (int c = 0; c < P; ++c) pu[c] = s[c];
+              } else {
+#pragma unroll
+                for (int c = 0; c < P; ++c) {
                   const long long dof = (long long)g[k] * P + c;
                   bool masked = false;
                   if (mask) masked = A.dmask[dof];
