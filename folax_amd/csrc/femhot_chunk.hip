@@ -220,7 +220,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
                                     p.perm_identity)
                     .total
               : cap4 * (3 + P + (mode == MODE_VJP ? P : 0) + (elemental ? 0 : 1)) * rs +
-                    (size_t)p.max_elems * c->n * (P + (nodal_gd ? 1 : 0)) * rs + (size_t)p.max_elems * c->n * 2 + 16;
+                    (size_t)p.max_elems * c->n * (P + (nodal_gd ? 1 : 0)) * rs + 16;
   if (shm > lds_limit()) {
     set_error("run_chunk: chunk too large for LDS; use fewer elements per chunk");
     return FEMHOT_ERR_UNSUPPORTED;
@@ -278,6 +278,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   A.lconn = p.lconn;
   A.inc = p.inc;
   A.inc_start = p.inc_start;
+  A.rank = p.rank;
   A.xyz = dtype == FEMHOT_F32 ? (const void *)p.chunk_xyz32 : (const void *)p.chunk_xyz64;
   A.n_chunks = p.n_chunks;
   if (cfg.sb > 0) {

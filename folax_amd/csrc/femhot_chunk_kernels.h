@@ -8,6 +8,7 @@ namespace femhot {
 struct ChunkArgs {
   const int *elem_off, *node_off, *perm, *nodes, *node_slot;
   const uint16_t *lconn, *inc, *inc_start;
+  const uint16_t *rank;  // per corner: its position in the chunk's incidence list
   const void *xyz;  // chunk-relative coordinates in the arithmetic type
   int n_chunks, n_sblocks, sb, batch, cap, emax;
   int perm_identity;  // perm[p] == p: the original element id is the position
@@ -93,12 +94,19 @@ template <class R, int K> __device__ __forceinline__ void store_corner_results(R
 //   U   [cap4 * P]      nodal solution
 //   W   [cap4 * P]      cotangent (VJP only)
 //   FD  [cap4]          nodal field (nodal fields only)
-//   ER  [emax * N * P]  per element corner ([position][corner][component]): residual / g_u.  An
-//                       incidence entry q = position * N + corner addresses its P values at q * P.
-//   ERD [emax * N]      per element corner ([position][corner]): nodal field-gradient contribution
-//   INC [emax * N] (uint16) the chunk's incidence list
+//   ER  [emax * N * P]  per element corner: residual / g_u, at [incidence-list position][component]
+//                       (the plan's rank of the corner), so each node's corners are one contiguous run
+//   ERD [emax * N]      per element corner, at its incidence-list position: nodal field-gradient
+//                       contribution
+// Waves per SIMD the compiler must fit: 4 for FP32 TET4 thermal and linear elasticity at 256 threads
+// (their LDS holds 4 workgroups of 512-tet chunks per CU, so more registers would only cost waves),
+// else its own choice.
+template <class R, int ET, int PH, int BLOCK> constexpr int chunk_min_waves() {
+  return (sizeof(R) == 4 && ET == TET4 && PH != NEOHOOKE && BLOCK == 256) ? 4 : 1;
+}
+
 template <class R, int ET, int PH, int NGP, int MODE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
+__global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void k_chunk(ChunkArgs A) {
   using E = Elem<R, ET, PH>;
   constexpr int N = E::N, D = E::D, P = E::P, NL = E::NL;
   // nodes / elements per lane per round of loads: enough that one round covers 512 nodes and
@@ -123,6 +131,9 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   const bool want_gd = (MODE != MODE_RES) && A.od != nullptr;
   const bool want_ou = A.ou != nullptr;
   const bool nodal_gd = want_gd && !elemental;
+  // Corner results go to ER / ERD at their position in the chunk's incidence list (the plan's
+  // per-corner rank), so a node sums a contiguous run and the gather reads no list.
+  const bool by_rank = want_ou || nodal_gd;
   const int cap4 = (A.cap + 3) & ~3, emax = A.emax;
   R *X = reinterpret_cast<R *>(smem_raw);
   R *U = X + 3 * cap4;
@@ -130,7 +141,6 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
   R *FD = W + (MODE == MODE_VJP ? P * cap4 : 0);
   R *ER = FD + (elemental ? 0 : cap4);
   R *ERD = ER + (long)emax * N * P;
-  uint16_t *INC = reinterpret_cast<uint16_t *>(ERD + (nodal_gd ? (long)emax * N : 0));
   const int tid = threadIdx.x;
   const int n_items = A.n_chunks * A.n_sblocks;
   using UN = typename std::conditional<N == 4, uint2, uint4>::type;  // one element's N uint16
@@ -163,7 +173,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       js0[k] = A.inc_start[n0 + i];
       je0[k] = (i + 1 < nl) ? (int)A.inc_start[n0 + i + 1] : N * ne;
     }
-    {  // coordinates (one 16-/32-byte record per node) and the incidence list
+    {  // coordinates (one 16-/32-byte record per node)
       const R *xyz = reinterpret_cast<const R *>(A.xyz) + 4LL * n0;
       for (int i = tid; i < nl; i += BLOCK) {
         if constexpr (sizeof(R) == 4) {
@@ -178,10 +188,6 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
           X[3 * i + 1] = v0.y;
           X[3 * i + 2] = v1.x;
         }
-      }
-      if (want_ou || nodal_gd) {  // one element's worth (N entries, 8 or 16 bytes) per access
-        const UN *inc = reinterpret_cast<const UN *>(A.inc + (long long)N * e0);
-        for (int j = tid; j < ne; j += BLOCK) reinterpret_cast<UN *>(INC)[j] = inc[j];
       }
     }
     const int b0 = sbi * A.sb, b1 = min(A.batch, b0 + A.sb);
@@ -219,13 +225,14 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       load_round(g0);
       // the first round of elements' local connectivity, original ids and per-element field, in
       // flight together with the solution gather above
-      UN lw0[KE];
+      UN lw0[KE], rk0[KE];
       int ee0[KE];
       R dev0[KE];
 #pragma unroll
       for (int k = 0; k < KE; ++k) {
         const int p = min(e0 + k * BLOCK + tid, e1 - 1);
         lw0[k] = *reinterpret_cast<const UN *>(A.lconn + (long long)p * N);
+        if (by_rank) rk0[k] = *reinterpret_cast<const UN *>(A.rank + (long long)p * N);
         ee0[k] = A.perm_identity ? p : A.perm[p];
       }
 #pragma unroll
@@ -246,12 +253,13 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
       double en = 0.0;
       R *ODb = reinterpret_cast<R *>(A.od) + (long long)b * A.ods;
       for (int base = e0; base < e1; base += KE * BLOCK) {
-        int l[KE][N], ee[KE];
+        int l[KE][N], rk[KE][N], ee[KE];
         R dev[KE];
         if (PF && base == e0) {
 #pragma unroll
           for (int k = 0; k < KE; ++k) {
             unpack_lconn<N>(lw0[k], l[k]);
+            if (by_rank) unpack_lconn<N>(rk0[k], rk[k]);
             ee[k] = ee0[k];
             dev[k] = dev0[k];
           }
@@ -260,6 +268,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
           for (int k = 0; k < KE; ++k) {
             const int p = min(base + k * BLOCK + tid, e1 - 1);
             load_lconn<N>(A.lconn + (long long)p * N, l[k]);
+            if (by_rank) load_lconn<N>(A.rank + (long long)p * N, rk[k]);
             ee[k] = A.perm_identity ? p : A.perm[p];
           }
           if (elemental) {
@@ -312,7 +321,12 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             elem_egrad<R, ET, PH, NGP>(el, A.pp, ro, gd, pi);
             en += (double)pi;
           }
-          if (want_ou) store_corner_results<R, NL>(ER + (long)pos * NL, ro);
+          if (want_ou) {  // each corner at its place in the incidence list
+#pragma unroll
+            for (int a = 0; a < N; ++a)
+#pragma unroll
+              for (int c = 0; c < P; ++c) ER[rk[k][a] * P + c] = ro[a * P + c];
+          }
           if constexpr (MODE != MODE_RES) {
             if (want_gd) {
               if (elemental) {
@@ -321,7 +335,8 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
                 else
                   ODb[ee[k]] = gd[0];
               } else {
-                store_corner_results<R, N>(ERD + (long)pos * N, gd);
+#pragma unroll
+                for (int a = 0; a < N; ++a) ERD[rk[k][a]] = gd[a];
               }
             }
           }
@@ -366,15 +381,11 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
               R s[P];
 #pragma unroll
               for (int c = 0; c < P; ++c) s[c] = R(-0.0);
-              // the next entry is read while this one's corner values load (the list is non-empty
-              // for every chunk node, so index je - 1 is always in the list)
-              int q = INC[js[k]];
+              // the node's corner results lie at its list positions js .. je - 1
               for (int j = js[k]; j < je[k]; ++j) {
-                const int qn = INC[min(j + 1, je[k] - 1)];
-                const R *e = ER + q * P;
+                const R *e = ER + j * P;
 #pragma unroll
                 for (int c = 0; c < P; ++c) s[c] += e[c];
-                q = qn;
               }
               if (sl[k] >= 0) {  // a shared node: its partial slot, the P values stored back to back
                 R *pu = PU + (long long)sl[k] * P;
@@ -396,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk(ChunkArgs A) {
             }
             if (nodal_gd) {
               R s = R(0);
-              for (int j = js[k]; j < je[k]; ++j) s += ERD[INC[j]];
+              for (int j = js[k]; j < je[k]; ++j) s += ERD[j];
               if (sl[k] >= 0)
                 PD[sl[k]] = s;
               else if (accumulate)

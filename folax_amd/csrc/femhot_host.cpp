@@ -195,6 +195,7 @@ static void free_chunks(ChunkPlan &p) {
   (void)hipFree(p.seg_chunk_off);
   (void)hipFree(p.inc);
   (void)hipFree(p.inc_start);
+  (void)hipFree(p.rank);
   p = ChunkPlan{};
 }
 
@@ -292,7 +293,7 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
     return sx != sy ? sx < sy : x.first < y.first;
   });
   std::vector<int> perm(Ne), elem_off{0}, node_off{0}, seg, nodes;
-  std::vector<uint16_t> lconn((size_t)Ne * n), inc((size_t)Ne * n), inc_start;
+  std::vector<uint16_t> lconn((size_t)Ne * n), inc((size_t)Ne * n), rank((size_t)Ne * n), inc_start;
   std::vector<int> cnt;
   std::vector<float> xyz32;
   std::vector<double> xyz64;
@@ -360,6 +361,7 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
     for (long long k = i; k < j; ++k)
       for (int a = 0; a < n; ++a) {
         const int q = lconn[(size_t)k * n + a];
+        rank[(size_t)k * n + a] = (uint16_t)cnt[q];  // inc[i * n + rank] is this corner
         inc[(size_t)i * n + cnt[q]++] = (uint16_t)((k - i) * n + a);
       }
     const double *ref = &c->h_coords[(size_t)loc[0] * 3];
@@ -422,7 +424,7 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
       (rc = upload(&p.node_slot, node_slot)) || (rc = upload(&p.sh_nodes, sh_nodes)) ||
       (rc = upload(&p.red_off, red_off)) ||
       (rc = upload(&p.seg_chunk_off, seg_chunk_off)) || (rc = upload(&p.inc, inc)) ||
-      (rc = upload(&p.inc_start, inc_start))) {
+      (rc = upload(&p.inc_start, inc_start)) || (rc = upload(&p.rank, rank))) {
     free_chunks(p);
     return rc;
   }

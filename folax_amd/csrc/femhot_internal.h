@@ -63,6 +63,7 @@ struct ChunkPlan {
   // entries touching it, sorted by element position; per chunk the list has n * elements entries
   uint16_t *inc = nullptr;        // (Ne * n), chunk by chunk
   uint16_t *inc_start = nullptr;  // (chunk nodes) start of each node's entries within its chunk
+  uint16_t *rank = nullptr;       // (Ne * n) position of corner (element position, a) in its chunk's list
   unsigned long long last_use = 0;  // plan cache: least recently used plan is evicted first
 };
 
