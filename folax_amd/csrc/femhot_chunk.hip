@@ -279,7 +279,8 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   A.inc = p.inc;
   A.inc_start = p.inc_start;
   A.rank = p.rank;
-  A.xyz = dtype == FEMHOT_F32 ? (const void *)p.chunk_xyz32 : (const void *)p.chunk_xyz64;
+  A.xyz = cfg.dma ? (const void *)p.chunk_xyz32
+                  : dtype == FEMHOT_F32 ? (const void *)p.chunk_xyz12 : (const void *)p.chunk_xyz64;
   A.n_chunks = p.n_chunks;
   if (cfg.sb > 0) {
     A.sb = std::min(cfg.sb, batch);

@@ -173,21 +173,13 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
       js0[k] = A.inc_start[n0 + i];
       je0[k] = (i + 1 < nl) ? (int)A.inc_start[n0 + i + 1] : N * ne;
     }
-    {  // coordinates (one 16-/32-byte record per node)
-      const R *xyz = reinterpret_cast<const R *>(A.xyz) + 4LL * n0;
+    {  // coordinates (one (x, y, z) record per node)
+      const R *xyz = reinterpret_cast<const R *>(A.xyz) + 3LL * n0;
       for (int i = tid; i < nl; i += BLOCK) {
-        if constexpr (sizeof(R) == 4) {
-          const float4 v = reinterpret_cast<const float4 *>(xyz)[i];
-          X[3 * i] = v.x;
-          X[3 * i + 1] = v.y;
-          X[3 * i + 2] = v.z;
-        } else {
-          const double2 v0 = reinterpret_cast<const double2 *>(xyz)[2 * i];
-          const double2 v1 = reinterpret_cast<const double2 *>(xyz)[2 * i + 1];
-          X[3 * i] = v0.x;
-          X[3 * i + 1] = v0.y;
-          X[3 * i + 2] = v1.x;
-        }
+        const R x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        X[3 * i] = x;
+        X[3 * i + 1] = y;
+        X[3 * i + 2] = z;
       }
     }
     const int b0 = sbi * A.sb, b1 = min(A.batch, b0 + A.sb);

@@ -187,6 +187,7 @@ static void free_chunks(ChunkPlan &p) {
   (void)hipFree(p.lconn);
   (void)hipFree(p.chunk_nodes);
   (void)hipFree(p.chunk_xyz32);
+  (void)hipFree(p.chunk_xyz12);
   (void)hipFree(p.chunk_xyz64);
   (void)hipFree(p.zero_nodes);
   (void)hipFree(p.node_slot);
@@ -253,7 +254,7 @@ static double chunk_plan_cost(const Ctx *c, std::vector<int> &order, int epc, co
     bytes += 128.0 * (double)lu;
     i = j;
   }
-  bytes += 26.0 * (double)chunk_nodes;
+  bytes += 22.0 * (double)chunk_nodes;
   for (long long g = 0; g < c->Nn; ++g)
     if (touch[g] == 1)
       bytes += 12.0;
@@ -295,7 +296,7 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
   std::vector<int> perm(Ne), elem_off{0}, node_off{0}, seg, nodes;
   std::vector<uint16_t> lconn((size_t)Ne * n), inc((size_t)Ne * n), rank((size_t)Ne * n), inc_start;
   std::vector<int> cnt;
-  std::vector<float> xyz32;
+  std::vector<float> xyz32, xyz12;
   std::vector<double> xyz64;
   // element order: spatial (Morton) chunks, or runs of the caller's own order when that order is
   // local enough to move fewer bytes (structured and mesher-ordered meshes often are)
@@ -366,10 +367,10 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
       }
     const double *ref = &c->h_coords[(size_t)loc[0] * 3];
     for (int g : loc)
-      for (int k = 0; k < 4; ++k) {  // 16-byte (32-byte for FP64) record per node: x, y, z, 0
+      for (int k = 0; k < 4; ++k) {  // (x, y, z) per node; the 16-byte FP32 record adds a 0
         const double v = k < 3 ? c->h_coords[(size_t)g * 3 + k] - ref[k] : 0.0;
-        xyz64.push_back(v);
         xyz32.push_back((float)v);
+        if (k < 3) xyz12.push_back((float)v), xyz64.push_back(v);
       }
     nodes.insert(nodes.end(), loc.begin(), loc.end());
     max_nodes = std::max<int>(max_nodes, (int)loc.size());
@@ -420,6 +421,7 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
   if ((rc = upload(&p.chunk_elem_off, elem_off)) || (rc = upload(&p.chunk_node_off, node_off)) ||
       (rc = upload(&p.chunk_seg, seg)) || (rc = upload(&p.perm, perm)) || (rc = upload(&p.lconn, lconn)) ||
       (rc = upload(&p.chunk_nodes, nodes)) || (rc = upload(&p.chunk_xyz32, xyz32)) ||
+      (rc = upload(&p.chunk_xyz12, xyz12)) ||
       (rc = upload(&p.chunk_xyz64, xyz64)) || (rc = upload(&p.zero_nodes, zero)) ||
       (rc = upload(&p.node_slot, node_slot)) || (rc = upload(&p.sh_nodes, sh_nodes)) ||
       (rc = upload(&p.red_off, red_off)) ||

@@ -50,7 +50,9 @@ struct ChunkPlan {
   uint16_t *lconn = nullptr;      // (Ne * n) chunk-local node indices, position order
   int *chunk_nodes = nullptr;     // global node id | 0x80000000 when shared
   float *chunk_xyz32 = nullptr;   // per chunk node: (x, y, z, 0) relative to the chunk's first node
-  double *chunk_xyz64 = nullptr;
+                                  // (16-byte records for the LDS-DMA kernel)
+  float *chunk_xyz12 = nullptr;   // per chunk node: (x, y, z), 12-byte records for the chunk kernel
+  double *chunk_xyz64 = nullptr;  // per chunk node: (x, y, z), 24-byte records
   long long n_zero = 0;           // nodes in no element (their outputs are cleared separately)
   int *zero_nodes = nullptr;
   // deterministic reduction of shared nodes: each (chunk, shared node) pair owns a partial slot
