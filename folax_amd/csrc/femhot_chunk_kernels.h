@@ -410,7 +410,10 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
           }
         }
       }
-      __syncthreads();
+      // No barrier here, so a wave done with its gather goes on to the next sample's or item's loads.
+      // That staging writes X, U, W and FD only, whose last reads (the element phase) precede the
+      // barrier above; ER and ERD are rewritten only after the next staging barrier, which every
+      // wave reaches after its gather; `red` is rewritten after that barrier too.
     }
   }
 }
