@@ -38,6 +38,7 @@ EXPORTED = [
     "femhot_energy_grad", "femhot_pcg", "femhot_spmv", "femhot_orient_elements", "femhot_autotune",
     "femhot_kernel_timing", "femhot_set_chunk_config", "femhot_residual_shape_vjp", "femhot_newton",
     "femhot_gauss_point_data", "femhot_energy_grad_noalloc", "femhot_pcg_async", "femhot_newton_async",
+    "femhot_chunk_plan_check",
 ]
 
 OP_RESIDUAL, OP_VJP, OP_ENERGY_GRAD, OP_TANGENT = 0, 1, 2, 3
@@ -114,6 +115,7 @@ def lib():
     L.femhot_autotune.argtypes = [vp, i32, i32, i32, vp, ll, vp, ll, vp, ll, vp, ll, vp, ll, vp, u32,
                                    C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_float), vp]
     L.femhot_orient_elements.argtypes = [i32, ll, C.POINTER(dbl), ll, C.POINTER(i32), C.POINTER(ll)]
+    L.femhot_chunk_plan_check.argtypes = [C.POINTER(MeshDesc), i32, C.POINTER(ll)]
     L.femhot_kernel_timing.argtypes = [vp, i32, C.POINTER(dbl), C.POINTER(ll)]
     L.femhot_set_chunk_config.argtypes = [vp, i32, i32, i32, u32, i32, i32, i32]
     for name in EXPORTED:
@@ -138,3 +140,33 @@ def orient_elements(elem_type, coords, conn):
                                        coords.ctypes.data_as(C.POINTER(C.c_double)), conn.shape[0],
                                        conn.ctypes.data_as(C.POINTER(C.c_int)), C.byref(n_sw)))
     return conn, int(n_sw.value)
+
+
+PLAN_STATS = ("n_chunks", "max_nodes", "max_elems", "chunk_nodes", "shared_nodes", "slots", "zero_nodes",
+              "caller_order")
+
+
+def chunk_plan_check(elem_type, coords, conn, epc, segment_offsets=None):
+    """Test hook: the chunk plan of `epc` elements per chunk built on the host (no GPU) and checked
+    against every invariant the chunk kernels index by; raises FemhotError naming the first violation.
+    Returns the plan's figures as a dict (keys PLAN_STATS)."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    if coords.shape[1] == 2:
+        coords = np.concatenate([coords, np.zeros((coords.shape[0], 1))], axis=1)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    d = MeshDesc()
+    d.elem_type, d.physics, d.num_gp = int(elem_type), THERMAL, 0
+    d.n_nodes, d.n_elems = coords.shape[0], conn.shape[0]
+    d.coords = coords.ctypes.data_as(C.POINTER(C.c_double))
+    d.conn = conn.ctypes.data_as(C.POINTER(C.c_int))
+    d.n_dirichlet = 0
+    so = None
+    if segment_offsets is not None:
+        so = np.ascontiguousarray(segment_offsets, dtype=np.int64)
+        d.n_segments = so.shape[0] - 1
+        d.segment_offsets = so.ctypes.data_as(C.POINTER(C.c_longlong))
+    else:
+        d.n_segments = 1
+    stats = (C.c_longlong * 8)()
+    check(lib().femhot_chunk_plan_check(C.byref(d), int(epc), stats))
+    return dict(zip(PLAN_STATS, [int(v) for v in stats]))

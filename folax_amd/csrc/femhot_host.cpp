@@ -263,7 +263,18 @@ static double chunk_plan_cost(const Ctx *c, std::vector<int> &order, int epc, co
   return bytes;
 }
 
-static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
+// Host side of a chunk plan: the arrays build_chunk_plan uploads, and the figures the self-check
+// (femhot_chunk_plan_check) reports.
+struct HostChunkPlan {
+  int epc = 0, n_chunks = 0, max_nodes = 0, max_elems = 0;
+  bool perm_identity = false;
+  std::vector<int> perm, elem_off, node_off, seg, nodes, zero, sh_nodes, node_slot, red_off, seg_chunk_off;
+  std::vector<uint16_t> lconn, inc, rank, inc_start;
+  std::vector<float> xyz32, xyz12;
+  std::vector<double> xyz64;
+};
+
+static int plan_chunks_host(const Ctx *c, int epc, HostChunkPlan &h) {
   const int n = c->n, D = c->dim;
   const long long Ne = c->Ne;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -293,11 +304,18 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
     const int sx = seg_of[x.second], sy = seg_of[y.second];
     return sx != sy ? sx < sy : x.first < y.first;
   });
-  std::vector<int> perm(Ne), elem_off{0}, node_off{0}, seg, nodes;
-  std::vector<uint16_t> lconn((size_t)Ne * n), inc((size_t)Ne * n), rank((size_t)Ne * n), inc_start;
+  h.epc = epc;
+  h.perm.assign(Ne, 0);
+  h.elem_off = {0};
+  h.node_off = {0};
+  h.lconn.assign((size_t)Ne * n, 0);
+  h.inc.assign((size_t)Ne * n, 0);
+  h.rank.assign((size_t)Ne * n, 0);
+  std::vector<int> &perm = h.perm, &elem_off = h.elem_off, &node_off = h.node_off, &seg = h.seg, &nodes = h.nodes;
+  std::vector<uint16_t> &lconn = h.lconn, &inc = h.inc, &rank = h.rank, &inc_start = h.inc_start;
   std::vector<int> cnt;
-  std::vector<float> xyz32, xyz12;
-  std::vector<double> xyz64;
+  std::vector<float> &xyz32 = h.xyz32, &xyz12 = h.xyz12;
+  std::vector<double> &xyz64 = h.xyz64;
   // element order: spatial (Morton) chunks, or runs of the caller's own order when that order is
   // local enough to move fewer bytes (structured and mesher-ordered meshes often are)
   {
@@ -305,8 +323,8 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
     for (long long e = 0; e < Ne; ++e) morton[e] = key[e].second, ident[e] = (int)e;
     const double cost_m = chunk_plan_cost(c, morton, epc, seg_of, false);
     const double cost_i = chunk_plan_cost(c, ident, epc, seg_of, true);
-    p.perm_identity = cost_i <= cost_m;
-    perm = p.perm_identity ? ident : morton;
+    h.perm_identity = cost_i <= cost_m;
+    perm = h.perm_identity ? ident : morton;
   }
   long long i = 0;
   std::vector<int> loc, loc2, lid, icount, lorder, lrank;
@@ -382,7 +400,7 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
   }
   std::vector<int> touch(c->Nn, 0);
   for (int g : nodes) touch[g]++;
-  std::vector<int> zero, sh_nodes;
+  std::vector<int> &zero = h.zero, &sh_nodes = h.sh_nodes;
   std::vector<int> sh_index(c->Nn, -1);
   for (long long g = 0; g < c->Nn; ++g) {
     if (touch[g] == 0) zero.push_back((int)g);
@@ -393,8 +411,9 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
   }
   // partial slots, node-major: the slots of shared node k are red_off[k] .. red_off[k+1] - 1, one per
   // chunk containing it, in chunk order (so the reduction reads contiguous memory)
-  std::vector<int> node_slot(nodes.size(), -1);
-  std::vector<int> red_off(sh_nodes.size() + 1, 0);
+  h.node_slot.assign(nodes.size(), -1);
+  h.red_off.assign(sh_nodes.size() + 1, 0);
+  std::vector<int> &node_slot = h.node_slot, &red_off = h.red_off;
   for (int g : sh_nodes) red_off[sh_index[g] + 1] = touch[g];
   for (size_t k = 0; k < sh_nodes.size(); ++k) red_off[k + 1] += red_off[k];
   {
@@ -404,33 +423,154 @@ static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
       if (touch[g] > 1) node_slot[k] = fillpos[sh_index[g]]++;
     }
   }
-  std::vector<int> seg_chunk_off(c->nseg + 1, 0);
+  h.seg_chunk_off.assign(c->nseg + 1, 0);
+  std::vector<int> &seg_chunk_off = h.seg_chunk_off;
   for (int s : seg) seg_chunk_off[s + 1]++;
   for (int s = 0; s < c->nseg; ++s) seg_chunk_off[s + 1] += seg_chunk_off[s];
-  p.epc = epc;
-  p.n_chunks = (int)seg.size();
-  p.max_nodes = max_nodes;
-  p.max_elems = max_elems;
-  p.n_shared = (long long)sh_nodes.size();
-  p.n_zero = (long long)zero.size();
-  p.n_slots = (long long)red_off.back();
+  h.n_chunks = (int)seg.size();
+  h.max_nodes = max_nodes;
+  h.max_elems = max_elems;
   // the pipelined kernel reads inc_start as whole 32-bit words: one entry of padding past the end
   inc_start.push_back(0);
   inc_start.push_back(0);
+  return FEMHOT_OK;
+}
+
+static int build_chunk_plan(Ctx *c, int epc, ChunkPlan &p) {
+  HostChunkPlan h;
+  if (int rc = plan_chunks_host(c, epc, h)) return rc;
+  p.epc = epc;
+  p.n_chunks = h.n_chunks;
+  p.max_nodes = h.max_nodes;
+  p.max_elems = h.max_elems;
+  p.perm_identity = h.perm_identity;
+  p.n_shared = (long long)h.sh_nodes.size();
+  p.n_zero = (long long)h.zero.size();
+  p.n_slots = (long long)h.red_off.back();
   int rc;
-  if ((rc = upload(&p.chunk_elem_off, elem_off)) || (rc = upload(&p.chunk_node_off, node_off)) ||
-      (rc = upload(&p.chunk_seg, seg)) || (rc = upload(&p.perm, perm)) || (rc = upload(&p.lconn, lconn)) ||
-      (rc = upload(&p.chunk_nodes, nodes)) || (rc = upload(&p.chunk_xyz32, xyz32)) ||
-      (rc = upload(&p.chunk_xyz12, xyz12)) ||
-      (rc = upload(&p.chunk_xyz64, xyz64)) || (rc = upload(&p.zero_nodes, zero)) ||
-      (rc = upload(&p.node_slot, node_slot)) || (rc = upload(&p.sh_nodes, sh_nodes)) ||
-      (rc = upload(&p.red_off, red_off)) ||
-      (rc = upload(&p.seg_chunk_off, seg_chunk_off)) || (rc = upload(&p.inc, inc)) ||
-      (rc = upload(&p.inc_start, inc_start)) || (rc = upload(&p.rank, rank))) {
+  if ((rc = upload(&p.chunk_elem_off, h.elem_off)) || (rc = upload(&p.chunk_node_off, h.node_off)) ||
+      (rc = upload(&p.chunk_seg, h.seg)) || (rc = upload(&p.perm, h.perm)) || (rc = upload(&p.lconn, h.lconn)) ||
+      (rc = upload(&p.chunk_nodes, h.nodes)) || (rc = upload(&p.chunk_xyz32, h.xyz32)) ||
+      (rc = upload(&p.chunk_xyz12, h.xyz12)) || (rc = upload(&p.chunk_xyz64, h.xyz64)) ||
+      (rc = upload(&p.zero_nodes, h.zero)) || (rc = upload(&p.node_slot, h.node_slot)) ||
+      (rc = upload(&p.sh_nodes, h.sh_nodes)) || (rc = upload(&p.red_off, h.red_off)) ||
+      (rc = upload(&p.seg_chunk_off, h.seg_chunk_off)) || (rc = upload(&p.inc, h.inc)) ||
+      (rc = upload(&p.inc_start, h.inc_start)) || (rc = upload(&p.rank, h.rank))) {
     free_chunks(p);
     return rc;
   }
   return FEMHOT_OK;
+}
+
+// Every invariant the chunk kernels rely on, checked on a host plan; "" when all hold, else the first
+// violation.  The kernels index LDS and the partial slots with these arrays unchecked.
+static std::string check_chunk_plan(const Ctx &c, const HostChunkPlan &h) {
+  const int n = c.n;
+  const long long Ne = c.Ne, Nn = c.Nn;
+  auto at = [](const char *what, long long i) { return std::string(what) + " (at " + std::to_string(i) + ")"; };
+  const int K = h.n_chunks;
+  if ((int)h.elem_off.size() != K + 1 || (int)h.node_off.size() != K + 1 || (int)h.seg.size() != K)
+    return "chunk offset arrays do not have n_chunks + 1 entries";
+  if (h.elem_off[0] != 0 || h.elem_off[K] != Ne) return "element offsets do not span [0, n_elems)";
+  if (h.node_off[0] != 0 || (size_t)h.node_off[K] != h.nodes.size()) return "node offsets do not span the node list";
+  if ((long long)h.perm.size() != Ne || (long long)h.lconn.size() != Ne * n || (long long)h.rank.size() != Ne * n ||
+      (long long)h.inc.size() != Ne * n)
+    return "per-element arrays have the wrong length";
+  if (h.inc_start.size() != h.nodes.size() + 2) return "incidence starts: one entry per chunk node plus 2 of padding";
+  if (h.xyz64.size() != 3 * h.nodes.size() || h.xyz12.size() != 3 * h.nodes.size() ||
+      h.xyz32.size() != 4 * h.nodes.size())
+    return "coordinate records have the wrong length";
+  std::vector<char> seen(Ne, 0);
+  for (long long e = 0; e < Ne; ++e) {
+    const int q = h.perm[e];
+    if (q < 0 || q >= Ne || seen[q]) return at("perm is not a permutation", e);
+    if (h.perm_identity && q != e) return at("perm_identity set but perm[i] != i", e);
+    seen[q] = 1;
+  }
+  std::vector<int> touch(Nn, 0), last_chunk(Nn, -1), icount, hit;
+  int max_nodes = 0, max_elems = 0;
+  for (int k = 0; k < K; ++k) {
+    const int e0 = h.elem_off[k], e1 = h.elem_off[k + 1], n0 = h.node_off[k], n1 = h.node_off[k + 1];
+    const int ne = e1 - e0, nl = n1 - n0, s = h.seg[k];
+    if (ne <= 0 || ne > h.epc) return at("chunk size outside [1, epc]", k);
+    if (nl <= 0) return at("chunk without nodes", k);
+    if (s < 0 || s >= c.nseg || (k > 0 && s < h.seg[k - 1])) return at("chunk segment out of order", k);
+    max_nodes = std::max(max_nodes, nl);
+    max_elems = std::max(max_elems, ne);
+    for (int i = n0; i < n1; ++i) {
+      const int g = h.nodes[i];
+      if (g < 0 || g >= Nn) return at("chunk node id out of range", i);
+      if (last_chunk[g] == k) return at("node listed twice in one chunk", i);
+      last_chunk[g] = k;
+      touch[g]++;
+      const double *ref = &c.h_coords[(size_t)h.nodes[n0] * 3];
+      for (int d = 0; d < 3; ++d) {
+        const double v = c.h_coords[(size_t)g * 3 + d] - ref[d];
+        if (h.xyz64[(size_t)i * 3 + d] != v || h.xyz12[(size_t)i * 3 + d] != (float)v ||
+            h.xyz32[(size_t)i * 4 + d] != (float)v)
+          return at("coordinate record differs from the chunk-relative coordinates", i);
+      }
+      if (h.xyz32[(size_t)i * 4 + 3] != 0.0f) return at("16-byte coordinate record not padded with 0", i);
+    }
+    icount.assign(nl, 0);
+    for (int e = e0; e < e1; ++e) {
+      const long long ge = h.perm[e];
+      if (ge < c.h_seg_off[s] || ge >= c.h_seg_off[s + 1]) return at("element outside its chunk's segment", e);
+      for (int a = 0; a < n; ++a) {
+        const int l = h.lconn[(size_t)e * n + a];
+        if (l >= nl) return at("local node index beyond the chunk's nodes", e);
+        if (h.nodes[n0 + l] != c.h_conn[ge * n + a]) return at("local connectivity names another node", e);
+        icount[l]++;
+      }
+    }
+    for (int q = 0; q < nl; ++q) {
+      if (icount[q] == 0) return at("chunk node in none of the chunk's elements", n0 + q);
+      if (q > 0 && icount[q] > icount[q - 1]) return at("chunk nodes not ordered by incident corners", n0 + q);
+      const int js = h.inc_start[n0 + q], je = q + 1 < nl ? h.inc_start[n0 + q + 1] : n * ne;
+      if (js != (q == 0 ? 0 : (int)h.inc_start[n0 + q - 1] + icount[q - 1]) || je - js != icount[q])
+        return at("incidence run does not match the node's corner count", n0 + q);
+    }
+    hit.assign((size_t)n * ne, 0);
+    for (int e = e0; e < e1; ++e)
+      for (int a = 0; a < n; ++a) {
+        const int l = h.lconn[(size_t)e * n + a], r = h.rank[(size_t)e * n + a];
+        const int js = h.inc_start[n0 + l], je = l + 1 < nl ? h.inc_start[n0 + l + 1] : n * ne;
+        if (r < js || r >= je) return at("corner rank outside its node's incidence run", e);
+        if (hit[r]++) return at("two corners share one incidence position", e);
+        if (h.inc[(size_t)e0 * n + r] != (e - e0) * n + a) return at("incidence entry does not name its corner", e);
+        if (r > js && h.inc[(size_t)e0 * n + r - 1] >= h.inc[(size_t)e0 * n + r])
+          return at("incidence run not in element-position order", e);
+      }
+  }
+  if (max_nodes != h.max_nodes || max_elems != h.max_elems) return "max_nodes / max_elems differ from the chunks";
+  for (int s = 0; s < c.nseg; ++s)
+    for (int k = h.seg_chunk_off[s]; k < h.seg_chunk_off[s + 1]; ++k)
+      if (h.seg[k] != s) return at("seg_chunk_off names a chunk of another segment", k);
+  if (h.seg_chunk_off[c.nseg] != K) return "seg_chunk_off does not end at n_chunks";
+  // zero / shared nodes, partial slots: a shared node owns one slot per chunk containing it, in
+  // chunk order, contiguous; exclusive nodes own none
+  size_t iz = 0, ish = 0;
+  for (long long g = 0; g < Nn; ++g) {
+    if (touch[g] == 0 && (iz >= h.zero.size() || h.zero[iz++] != g)) return at("zero_nodes misses a node", g);
+    if (touch[g] > 1) {
+      if (ish >= h.sh_nodes.size() || h.sh_nodes[ish] != g) return at("sh_nodes misses a shared node", g);
+      if (h.red_off[ish + 1] - h.red_off[ish] != touch[g]) return at("shared node's slot count", g);
+      ++ish;
+    }
+  }
+  if (iz != h.zero.size() || ish != h.sh_nodes.size()) return "zero_nodes / sh_nodes list extra nodes";
+  std::vector<int> sh_index(Nn, -1), next_slot(h.red_off.begin(), h.red_off.end() - 1);
+  for (size_t k = 0; k < h.sh_nodes.size(); ++k) sh_index[h.sh_nodes[k]] = (int)k;
+  for (int k = 0; k < K; ++k)
+    for (int i = h.node_off[k]; i < h.node_off[k + 1]; ++i) {
+      const int g = h.nodes[i], sl = h.node_slot[i];
+      if (touch[g] == 1) {
+        if (sl != -1) return at("exclusive node owns a partial slot", i);
+      } else if (sl != next_slot[sh_index[g]]++) {
+        return at("partial slots of a shared node not contiguous in chunk order", i);
+      }
+    }
+  return "";
 }
 
 // Plans are cached per elements-per-chunk, so call shapes tuned to different chunk sizes (a forward
@@ -551,14 +691,9 @@ static double det_at_center(int et, const double *X) {
 
 using namespace femhot;
 
-extern "C" {
-
-const char *femhot_last_error(void) { return get_error().c_str(); }
-const char *femhot_version(void) { return "femhot 0.1 gfx950"; }
-
-int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
-  FH_CHECK_ARG(d && out, FEMHOT_ERR_ARG, "null argument");
-  *out = nullptr;
+// Host-side part of a context: the checks on the mesh description, the host copies of the mesh and
+// the physics constants (no device memory).  On failure the error is set.
+static int init_host_ctx(const femhot_mesh_desc *d, Ctx &c) {
   FH_CHECK_ARG(d->elem_type >= QUAD4 && d->elem_type <= TET4, FEMHOT_ERR_ARG, "unknown element type");
   FH_CHECK_ARG(d->physics >= THERMAL && d->physics <= NEOHOOKE, FEMHOT_ERR_ARG, "unknown physics");
   FH_CHECK_ARG(d->num_gp >= 0 && d->num_gp <= 3, FEMHOT_ERR_ARG, "num_gp must be 0 (default), 1, 2 or 3");
@@ -567,8 +702,6 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
   FH_CHECK_ARG(d->coords && d->conn, FEMHOT_ERR_ARG, "null coords/conn");
   FH_CHECK_ARG(d->n_dirichlet == 0 || (d->dirichlet_idx && d->dirichlet_val), FEMHOT_ERR_ARG,
                "null Dirichlet arrays");
-  auto *h = new femhot_ctx();
-  Ctx &c = h->c;
   c.desc = *d;
   c.et = d->elem_type;
   c.ph = d->physics;
@@ -583,7 +716,6 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
   c.h_conn.assign(d->conn, d->conn + c.Ne * c.n);
   for (long long i = 0; i < c.Ne * c.n; ++i)
     if (c.h_conn[i] < 0 || c.h_conn[i] >= c.Nn) {
-      delete h;
       set_error("femhot_ctx_create: connectivity index out of range");
       return FEMHOT_ERR_INDEX;
     }
@@ -598,7 +730,6 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
         }
   for (long long i = 0; i < c.Nn * 3; ++i)
     if (!std::isfinite(c.h_coords[i])) {
-      delete h;
       set_error("femhot_ctx_create: non-finite coordinate");
       return FEMHOT_ERR_NONFINITE;
     }
@@ -606,14 +737,12 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
   c.h_dir_val.assign(d->dirichlet_val, d->dirichlet_val + d->n_dirichlet);
   for (long long v : c.h_dir_idx)
     if (v < 0 || v >= c.Ndof) {
-      delete h;
       set_error("femhot_ctx_create: Dirichlet dof index out of range");
       return FEMHOT_ERR_INDEX;
     }
   c.nseg = d->n_segments > 1 ? d->n_segments : 1;
   if (c.nseg > 1) {
     if (!d->segment_offsets) {
-      delete h;
       set_error("femhot_ctx_create: segment_offsets required when n_segments > 1");
       return FEMHOT_ERR_ARG;
     }
@@ -621,7 +750,6 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
     bool ok = c.h_seg_off[0] == 0 && c.h_seg_off[c.nseg] == c.Ne;
     for (int s = 0; s < c.nseg; ++s) ok = ok && c.h_seg_off[s] <= c.h_seg_off[s + 1];
     if (!ok) {
-      delete h;
       set_error("femhot_ctx_create: segment_offsets must be non-decreasing from 0 to n_elems");
       return FEMHOT_ERR_SHAPE;
     }
@@ -646,6 +774,23 @@ int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
   p.f2sh = p.f2 * (1.0 - nu) / 2.0;
   p.kap1 = 1.0 / (3.0 * (1.0 - 2.0 * nu));
   p.mu1 = 1.0 / (2.0 * (1.0 + nu));
+  return FEMHOT_OK;
+}
+
+extern "C" {
+
+const char *femhot_last_error(void) { return get_error().c_str(); }
+const char *femhot_version(void) { return "femhot 0.1 gfx950"; }
+
+int femhot_ctx_create(const femhot_mesh_desc *d, femhot_ctx **out) {
+  FH_CHECK_ARG(d && out, FEMHOT_ERR_ARG, "null argument");
+  *out = nullptr;
+  auto *h = new femhot_ctx();
+  Ctx &c = h->c;
+  if (int rc = init_host_ctx(d, c)) {
+    delete h;
+    return rc;
+  }
   // device copies
   std::vector<float> c32(c.h_coords.begin(), c.h_coords.end());
   std::vector<uint8_t> dmask(c.Ndof, 0);
@@ -768,6 +913,25 @@ int femhot_kernel_timing(femhot_ctx *h, int enable, double *total_ms, long long 
   c.tev_used = 0;
   if (total_ms) *total_ms = ms;
   if (launches) *launches = n;
+  return FEMHOT_OK;
+}
+
+int femhot_chunk_plan_check(const femhot_mesh_desc *d, int epc, long long *stats) {
+  FH_CHECK_ARG(d && stats, FEMHOT_ERR_ARG, "null argument");
+  FH_CHECK_ARG(epc > 0, FEMHOT_ERR_ARG, "elements per chunk must be positive");
+  Ctx c;
+  if (int rc = init_host_ctx(d, c)) return rc;
+  HostChunkPlan h;
+  if (int rc = plan_chunks_host(&c, epc, h)) return rc;
+  const std::string why = check_chunk_plan(c, h);
+  const long long v[8] = {h.n_chunks, h.max_nodes, h.max_elems, (long long)h.nodes.size(),
+                          (long long)h.sh_nodes.size(), (long long)h.red_off.back(), (long long)h.zero.size(),
+                          h.perm_identity ? 1 : 0};
+  for (int k = 0; k < 8; ++k) stats[k] = v[k];
+  if (!why.empty()) {
+    set_error("femhot_chunk_plan_check: " + why);
+    return FEMHOT_ERR_INDEX;
+  }
   return FEMHOT_OK;
 }
 

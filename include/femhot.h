@@ -302,6 +302,16 @@ int femhot_spmv(femhot_ctx *ctx, const long long *row_ptr, const int *col_idx, c
 int femhot_orient_elements(int elem_type, long long n_nodes, const double *coords, long long n_elems, int *conn,
                            long long *n_swapped);
 
+/* Test hook (no counterpart in the reference; for the library's own tests and host sanitizer runs):
+ * builds the chunk plan of `epc` elements per chunk for the mesh `desc` on the HOST only (no device
+ * memory, no GPU needed) and checks every invariant the chunk kernels index by: element
+ * permutation, segment runs, chunk-local connectivity against the mesh, incidence runs and per-corner
+ * ranks, coordinate records, zero / shared node lists and the partial-slot layout.  stats[0..7] =
+ * n_chunks, max nodes per chunk, max elements per chunk, staged chunk nodes, shared nodes, partial
+ * slots, nodes in no element, 1 if the chunks are runs of the caller's element order.  Returns
+ * FEMHOT_ERR_INDEX, with the first violation in femhot_last_error(), when an invariant fails. */
+int femhot_chunk_plan_check(const femhot_mesh_desc *desc, int epc, long long *stats);
+
 #ifdef __cplusplus
 }
 #endif
