@@ -168,7 +168,7 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
 #pragma unroll
     for (int k = 0; k < KN; ++k) {
       const int i = min(k * BLOCK + tid, nl - 1);
-      g0[k] = A.nodes[n0 + i] & 0x7fffffff;
+      g0[k] = A.nodes[n0 + i];
       sl0[k] = A.node_slot[n0 + i];
       js0[k] = A.inc_start[n0 + i];
       je0[k] = (i + 1 < nl) ? (int)A.inc_start[n0 + i + 1] : N * ne;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
       for (int base = KN * BLOCK; base < nl; base += KN * BLOCK) {  // chunks of more nodes
         int g[KN];
 #pragma unroll
-        for (int k = 0; k < KN; ++k) g[k] = A.nodes[n0 + min(base + k * BLOCK + tid, nl - 1)] & 0x7fffffff;
+        for (int k = 0; k < KN; ++k) g[k] = A.nodes[n0 + min(base + k * BLOCK + tid, nl - 1)];
         load_round(g);
         store_round(base);
       }
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
 #pragma unroll
             for (int k = 0; k < KN; ++k) {
               const int i = min(base + k * BLOCK + tid, nl - 1);
-              g[k] = A.nodes[n0 + i] & 0x7fffffff;
+              g[k] = A.nodes[n0 + i];
               sl[k] = A.node_slot[n0 + i];
               js[k] = A.inc_start[n0 + i];
               je[k] = (i + 1 < nl) ? (int)A.inc_start[n0 + i + 1] : N * ne;

@@ -170,7 +170,7 @@ __device__ __forceinline__ void pipe_issue_data(const ChunkArgs &A, const PipeLa
   for (int base = wv * 64; base < G.nl; base += BLOCK) {
     const int i = base + lane;
     if (i < G.nl) {
-      const long long g = nid[i] & 0x7fffffff;
+      const long long g = nid[i];
 #pragma unroll
       for (int c = 0; c < P; ++c) dma4(Ub + g * P + c, sm + L.u + (c * cap64 + base) * 4);
       if constexpr (MODE == MODE_VJP) {
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(BLOCK) void k_chunk_pipe(ChunkArgs A) {
       R *PU = reinterpret_cast<R *>(A.part_u) + (long long)b * A.n_slots * P;
       R *PD = reinterpret_cast<R *>(A.part_d) + (long long)b * A.n_slots;
       for (int i = tid; i < G.nl; i += BLOCK) {
-        const int g = NID[i] & 0x7fffffff;
+        const int g = NID[i];
         const int sl = SLOT[i];
         const int js = IST[i];
         const int je = (i + 1 < G.nl) ? (int)IST[i + 1] : N * G.ne;

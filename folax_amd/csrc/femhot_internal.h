@@ -48,7 +48,7 @@ struct ChunkPlan {
   int *perm = nullptr;            // (Ne) original element id at each position
   bool perm_identity = false;     // chunks are runs of the caller's element order (perm[i] == i)
   uint16_t *lconn = nullptr;      // (Ne * n) chunk-local node indices, position order
-  int *chunk_nodes = nullptr;     // global node id | 0x80000000 when shared
+  int *chunk_nodes = nullptr;     // global node id of each chunk node
   float *chunk_xyz32 = nullptr;   // per chunk node: (x, y, z, 0) relative to the chunk's first node
                                   // (16-byte records for the LDS-DMA kernel)
   float *chunk_xyz12 = nullptr;   // per chunk node: (x, y, z), 12-byte records for the chunk kernel
