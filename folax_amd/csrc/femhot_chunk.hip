@@ -3,14 +3,14 @@
 //
 // One workgroup owns one chunk of spatially compact elements at a time (grid-stride loop over
 // chunk x sample-block work items, so the launch can be persistent).  Per work item:
-//   1. stage the chunk's node coordinates (precomputed chunk-relative, contiguous -> coalesced),
-//      its incidence list, and the chunk's nodal solution / cotangent / field (node ids sorted)
-//      into LDS;
-//   2. one lane per element: element routine reading LDS; its per-corner results go to an LDS
-//      buffer (no atomics), per-element results straight to HBM;
-//   3. one lane per chunk node sums the node's corner results in a fixed order and writes them
-//      once with a plain store: to the output when the node belongs to this chunk only, to the
-//      node's partial slot when it is shared with other chunks.
+//   1. stage the chunk's node coordinates (precomputed chunk-relative, contiguous -> coalesced)
+//      and the chunk's nodal solution / cotangent / field into LDS;
+//   2. one lane per element: element routine reading LDS; each corner's results go to an LDS
+//      buffer at the corner's position in the chunk's node-sorted incidence list (the plan's
+//      per-corner rank; no atomics), per-element results straight to HBM;
+//   3. one lane per chunk node sums its contiguous run of that buffer in a fixed order and writes
+//      the sum once with a plain store: to the output when the node belongs to this chunk only, to
+//      the node's partial slot when it is shared with other chunks.
 // k_reduce then sums each shared node's partials in chunk order and the per-chunk energies per
 // segment, so the path uses no atomics (device-scope atomics resolve beyond the per-XCD L2) and
 // its results are bitwise reproducible run to run.
