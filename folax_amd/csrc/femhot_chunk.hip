@@ -215,12 +215,29 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
     set_error("run_chunk: the LDS-DMA chunk kernel is FP32 only");
     return FEMHOT_ERR_UNSUPPORTED;
   }
+  // The staged TET4 kernel's corner buffer takes one spare row per chunk node (a node gather without
+  // LDS bank conflicts) unless those rows would cost a workgroup per CU: compared in 512-byte LDS
+  // allocation units, against the workgroups per CU the configuration asks for (any number for one
+  // workgroup per item).
+  const size_t shm_base = cap4 * (3 + P + (mode == MODE_VJP ? P : 0) + (elemental ? 0 : 1)) * rs +
+                          (size_t)p.max_elems * c->n * (P + (nodal_gd ? 1 : 0)) * rs + 16;
+  const size_t shm_rows = cap4 * (P + (nodal_gd ? 1 : 0)) * rs;
+  bool pad = false;
+  if (!cfg.dma && c->et == TET4) {
+    const size_t lds_cu = lds_limit() + 256;
+    auto per_cu = [&](size_t b) {
+      const size_t alloc = (b + (size_t)cfg.block / 64 * sizeof(double) + 511) / 512 * 512;
+      return (long long)(lds_cu / alloc);
+    };
+    const long long want = cfg.grid_per_cu > 0 ? cfg.grid_per_cu : (1LL << 30);
+    pad = shm_base + shm_rows <= lds_limit() &&
+          per_cu(shm_base + shm_rows) >= std::min(per_cu(shm_base), want);
+  }
   const size_t shm =
       cfg.dma ? (size_t)pipe_layout(p.max_nodes, p.max_elems, c->n, P, mode, elemental, ou != nullptr, nodal_gd,
                                     p.perm_identity)
                     .total
-              : cap4 * (3 + P + (mode == MODE_VJP ? P : 0) + (elemental ? 0 : 1)) * rs +
-                    (size_t)p.max_elems * c->n * (P + (nodal_gd ? 1 : 0)) * rs + 16;
+              : shm_base + (pad ? shm_rows : 0);
   if (shm > lds_limit()) {
     set_error("run_chunk: chunk too large for LDS; use fewer elements per chunk");
     return FEMHOT_ERR_UNSUPPORTED;
@@ -293,6 +310,7 @@ static int run_chunk_cfg(Ctx *c, int dtype, int mode, int batch, const void *fie
   A.cap = p.max_nodes;
   A.emax = p.max_elems;
   A.perm_identity = p.perm_identity ? 1 : 0;
+  A.pad = pad ? 1 : 0;
   A.n_slots = p.n_slots;
   A.field = field;
   A.fs = batch > 1 ? fs : 0;

@@ -12,6 +12,9 @@ struct ChunkArgs {
   const void *xyz;  // chunk-relative coordinates in the arithmetic type
   int n_chunks, n_sblocks, sb, batch, cap, emax;
   int perm_identity;  // perm[p] == p: the original element id is the position
+  // 1: row r of node q's run sits at r + q in ER / ERD (one spare row per node), so that the lanes of
+  // a node-gather wave, whose runs all have about the same length, fall on different LDS banks (TET4)
+  int pad;
   long long n_slots;
   const void *field;
   long long fs;
@@ -94,10 +97,10 @@ template <class R, int K> __device__ __forceinline__ void store_corner_results(R
 //   U   [cap4 * P]      nodal solution
 //   W   [cap4 * P]      cotangent (VJP only)
 //   FD  [cap4]          nodal field (nodal fields only)
-//   ER  [emax * N * P]  per element corner: residual / g_u, at [incidence-list position][component]
-//                       (the plan's rank of the corner), so each node's corners are one contiguous run
-//   ERD [emax * N]      per element corner, at its incidence-list position: nodal field-gradient
-//                       contribution
+//   ER  [(emax * N + cap4 * pad) * P]  per element corner: residual / g_u, at [incidence-list position
+//                       (the plan's rank of the corner) + pad * chunk node][component], so each node's
+//                       corners are one contiguous run
+//   ERD [emax * N + cap4 * pad]  per element corner, at the same row: nodal field-gradient contribution
 // Waves per SIMD the compiler must fit: 4 for FP32 TET4 thermal and linear elasticity at 256 threads
 // (their LDS holds 4 workgroups of 512-tet chunks per CU, so more registers would only cost waves),
 // else its own choice.
@@ -140,7 +143,10 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
   R *W = U + P * cap4;
   R *FD = W + (MODE == MODE_VJP ? P * cap4 : 0);
   R *ER = FD + (elemental ? 0 : cap4);
-  R *ERD = ER + (long)emax * N * P;
+  // (TET4 only: its interior nodes' runs of 24 corners x 3 components put a wave's lanes 72 words
+  // apart, on 4 of 32 banks; QUAD4 and HEX8 measured no gain from the spare rows)
+  const int pad = ET == TET4 ? A.pad : 0;
+  R *ERD = ER + (long)(emax * N + cap4 * pad) * P;
   const int tid = threadIdx.x;
   const int n_items = A.n_chunks * A.n_sblocks;
   using UN = typename std::conditional<N == 4, uint2, uint4>::type;  // one element's N uint16
@@ -317,7 +323,7 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
 #pragma unroll
             for (int a = 0; a < N; ++a)
 #pragma unroll
-              for (int c = 0; c < P; ++c) ER[rk[k][a] * P + c] = ro[a * P + c];
+              for (int c = 0; c < P; ++c) ER[(rk[k][a] + pad * l[k][a]) * P + c] = ro[a * P + c];
           }
           if constexpr (MODE != MODE_RES) {
             if (want_gd) {
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
                   ODb[ee[k]] = gd[0];
               } else {
 #pragma unroll
-                for (int a = 0; a < N; ++a) ERD[rk[k][a]] = gd[a];
+                for (int a = 0; a < N; ++a) ERD[rk[k][a] + pad * l[k][a]] = gd[a];
               }
             }
           }
@@ -373,8 +379,8 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
               R s[P];
 #pragma unroll
               for (int c = 0; c < P; ++c) s[c] = R(-0.0);
-              // the node's corner results lie at its list positions js .. je - 1
-              for (int j = js[k]; j < je[k]; ++j) {
+              // the node's corner results lie at its list positions js .. je - 1 (+ i with the spare rows)
+              for (int j = js[k] + pad * i; j < je[k] + pad * i; ++j) {
                 const R *e = ER + j * P;
 #pragma unroll
                 for (int c = 0; c < P; ++c) s[c] += e[c];
@@ -399,7 +405,7 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
             }
             if (nodal_gd) {
               R s = R(0);
-              for (int j = js[k]; j < je[k]; ++j) s += ERD[j];
+              for (int j = js[k] + pad * i; j < je[k] + pad * i; ++j) s += ERD[j];
               if (sl[k] >= 0)
                 PD[sl[k]] = s;
               else if (accumulate)
