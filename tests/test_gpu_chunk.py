@@ -130,6 +130,48 @@ def test_chunk_uneven_meshes_element_field(dt):
     assert rel(en.cpu().numpy().ravel(), seg_ref) <= TOL[dt]
 
 
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_tet4_spare_rows_same_bits_across_grids(dt):
+    """The TET4 chunk kernel shifts each chunk node's corner rows by its local index (one spare row per
+    node) unless the rows would cost a workgroup per CU. On this mesh at 256 threads: the potential
+    gradient (per-element field, 512-tet chunks) takes the shift at 2 and 4 workgroups per CU and not at
+    5 or 8; the residual VJP (nodal field, so the field-gradient rows too; 256-tet chunks) at 2 and 6 and
+    not at 7 or 8. The shift moves no term of any sum, so every grid gives the same bits, and the
+    oracle's values."""
+    from folax_amd import _lib as L
+    from folax_amd.context import FemhotContext
+    coords, conn, _ = fe.box_tet_mesh(12, 1.0, jitter=0.2, seed=77)
+    conn = conn.astype(np.int32)
+    rng = np.random.default_rng(78)
+    Ee = np.exp(rng.uniform(0.0, np.log(100.0), size=conn.shape[0]))
+    En = np.exp(rng.uniform(0.0, np.log(100.0), size=coords.shape[0]))
+    u = rng.uniform(-0.01, 0.01, size=3 * coords.shape[0])
+    w = rng.uniform(-1.0, 1.0, size=3 * coords.shape[0])
+    ctx = FemhotContext("tetra", L.MECHANICAL, coords, conn, young_modulus=1.0, poisson_ratio=0.3)
+    outs = []
+    for grid, vgrid in ((2, 2), (4, 6), (5, 7), (8, 8)):
+        print(f"[spare rows] {dt} 512 x 256 x {grid}, 256 x 256 x {vgrid}", flush=True)
+        ctx.set_chunk_config(L.OP_ENERGY_GRAD, TDT[dt], 1, 512, 256, grid, flags=L.ELEMENT_FIELD)
+        ctx.set_chunk_config(L.OP_VJP, TDT[dt], 1, 256, 256, vgrid)
+        en, gu, gd = ctx.energy_grad(dev(Ee, dt)[None], dev(u, dt)[None], flags=CHUNK | L.ELEMENT_FIELD)
+        vu, vd = ctx.residual_vjp(dev(En, dt)[None], dev(u, dt)[None], dev(w, dt)[None], flags=CHUNK)
+        torch.cuda.synchronize()
+        outs.append([x.clone() for x in (en, gu, gd, vu, vd)])
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert torch.equal(a, b)
+    ph = fe.Physics("mechanical", "tetra", E=1.0, nu=0.3)
+    Pi, r, gde = fe.energy_grad(ph, coords[conn], Ee, u[fe.element_dofs(conn, 3)])
+    en, gu, gd, vu, vd = outs[0]
+    assert rel(gu.cpu().numpy(), fe.scatter_nodal(conn, r, coords.shape[0], 3)) <= TOL[dt]
+    assert rel(gd.cpu().numpy(), gde) <= TOL[dt]
+    assert rel(en.cpu().numpy().sum(), Pi.sum()) <= TOL[dt]
+    vu_ref, vd_ref = fe.global_residual_vjp(ph, coords, conn, En, u, w)
+    assert rel(vu.cpu().numpy(), vu_ref) <= TOL[dt]
+    assert rel(vd.cpu().numpy(), vd_ref) <= TOL[dt]
+    ctx.close()
+
+
 def test_chunk_flag_takes_plain_path_for_fp64_neohooke():
     """Untuned FEMHOT_SCATTER_CHUNK calls in FP64 Neo-Hookean take the plain path: same numbers as
     an explicit plain call, and as the chunk kernel when it is forced."""
