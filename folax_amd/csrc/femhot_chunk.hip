@@ -27,9 +27,11 @@
 namespace femhot {
 
 // Both deterministic reductions in one launch.  Blocks [0, nb_nodes) sum, for each shared node,
-// the partials of every chunk containing it (a node's slots are contiguous and in chunk order, so
-// consecutive threads read consecutive memory); the last nseg * batch blocks each sum the
-// per-chunk energies of one (sample, segment) in a fixed order.
+// the partials of every chunk containing it, in chunk order; the last nseg * batch blocks each sum
+// the per-chunk energies of one (sample, segment) in a fixed order.  Shared node k's partial from
+// the o-th chunk containing it is slot k (o = 0), n_shared + k (o = 1) or, for nodes in 3+ chunks,
+// 2 n_shared + red_off[k] - 2 k + o - 2: every shared node has the first two, and their addresses
+// need no table, so they load together with the node's ids (consecutive threads, consecutive memory).
 template <class R> struct ReduceArgs {
   long long n_shared, n_slots;
   int P, batch, nb_nodes, nseg, n_chunks;
@@ -71,46 +73,48 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> A) {
   const bool mask = (A.flags & FEMHOT_MASK_DIRICHLET) != 0;
   bool ok[REDUCE_NPT];
   long long bu[REDUCE_NPT];
-  int gu[REDUCE_NPT], j0u[REDUCE_NPT], nu[REDUCE_NPT];
+  int gu[REDUCE_NPT], nu[REDUCE_NPT];
+  long long ku[REDUCE_NPT], xu[REDUCE_NPT];
 #pragma unroll
   for (int u = 0; u < REDUCE_NPT; ++u) {
     const long long i = (long long)blockIdx.x * (256 * REDUCE_NPT) + u * 256 + threadIdx.x;
     ok[u] = i < tot;
     const long long ic = ok[u] ? i : tot - 1;  // clamped: the loads stay in range, the stores are skipped
     bu[u] = ic / A.n_shared;
-    const long long k = ic - bu[u] * A.n_shared;
-    gu[u] = A.sh_nodes[k];
-    j0u[u] = A.red_off[k];
-    nu[u] = A.red_off[k + 1] - j0u[u];  // a shared node has a slot in >= 2 chunks
+    ku[u] = ic - bu[u] * A.n_shared;
+    gu[u] = A.sh_nodes[ku[u]];
+    const int r0 = A.red_off[ku[u]];
+    nu[u] = A.red_off[ku[u] + 1] - r0;  // a shared node has a slot in >= 2 chunks
+    xu[u] = 2 * A.n_shared + r0 - 2 * ku[u] - 2;  // + o: its o-th slot, o >= 2
   }
-  // Every load a node needs is issued before the first add: its first two slots (consecutive
-  // memory), the Dirichlet mask and, when accumulating, the old outputs.  Further slots (nodes in
-  // 3+ chunks) follow in a loop.  The sum runs over the slots in chunk order.
+  // Every load a node needs is issued before the first add: its first two slots (at fixed places),
+  // the Dirichlet mask and, when accumulating, the old outputs.  Further slots (nodes in 3+ chunks)
+  // follow in a loop.  The sum runs over the slots in chunk order.
   if (A.ou) {
     R v0[REDUCE_NPT][P], v1[REDUCE_NPT][P], old[REDUCE_NPT][P];
     bool masked[REDUCE_NPT][P];
 #pragma unroll
     for (int u = 0; u < REDUCE_NPT; ++u) {
-      const R *pu = A.part_u + bu[u] * A.n_slots * P + (long long)j0u[u] * P;
+      const R *pb = A.part_u + bu[u] * A.n_slots * P;
       const R *o = A.ou + bu[u] * A.ous + (long long)gu[u] * P;
 #pragma unroll
       for (int c = 0; c < P; ++c) {
-        v0[u][c] = pu[c];
-        v1[u][c] = nu[u] >= 2 ? pu[P + c] : R(-0.0);
+        v0[u][c] = pb[ku[u] * P + c];
+        v1[u][c] = pb[(A.n_shared + ku[u]) * P + c];
         masked[u][c] = mask && A.dmask[(long long)gu[u] * P + c];
         old[u][c] = accumulate ? o[c] : R(0);
       }
     }
 #pragma unroll
     for (int u = 0; u < REDUCE_NPT; ++u) {
-      const R *pu = A.part_u + bu[u] * A.n_slots * P + (long long)j0u[u] * P;
+      const R *pb = A.part_u + bu[u] * A.n_slots * P;
       R *o = A.ou + bu[u] * A.ous + (long long)gu[u] * P;
       R s[P];
 #pragma unroll
       for (int c = 0; c < P; ++c) s[c] = (R(-0.0) + v0[u][c]) + v1[u][c];
       for (int j = 2; j < nu[u]; ++j)
 #pragma unroll
-        for (int c = 0; c < P; ++c) s[c] += pu[j * P + c];
+        for (int c = 0; c < P; ++c) s[c] += pb[(xu[u] + j) * P + c];
       if (ok[u]) {
 #pragma unroll
         for (int c = 0; c < P; ++c)
@@ -122,16 +126,16 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs<R> A) {
     R v0[REDUCE_NPT], v1[REDUCE_NPT], old[REDUCE_NPT];
 #pragma unroll
     for (int u = 0; u < REDUCE_NPT; ++u) {
-      const R *pd = A.part_d + bu[u] * A.n_slots + j0u[u];
-      v0[u] = pd[0];
-      v1[u] = nu[u] >= 2 ? pd[1] : R(-0.0);
+      const R *pd = A.part_d + bu[u] * A.n_slots;
+      v0[u] = pd[ku[u]];
+      v1[u] = pd[A.n_shared + ku[u]];
       old[u] = accumulate ? A.od[bu[u] * A.ods + gu[u]] : R(0);
     }
 #pragma unroll
     for (int u = 0; u < REDUCE_NPT; ++u) {
-      const R *pd = A.part_d + bu[u] * A.n_slots + j0u[u];
+      const R *pd = A.part_d + bu[u] * A.n_slots;
       R s = (R(-0.0) + v0[u]) + v1[u];
-      for (int j = 2; j < nu[u]; ++j) s += pd[j];
+      for (int j = 2; j < nu[u]; ++j) s += pd[xu[u] + j];
       if (ok[u]) A.od[bu[u] * A.ods + gu[u]] = accumulate ? old[u] + s : s;
     }
   }

@@ -150,9 +150,9 @@ __global__ __launch_bounds__(BLOCK, (chunk_min_waves<R, ET, PH, BLOCK>())) void 
   const int tid = threadIdx.x;
   const int n_items = A.n_chunks * A.n_sblocks;
   using UN = typename std::conditional<N == 4, uint2, uint4>::type;  // one element's N uint16
-  // Workgroups go to the 8 XCDs in turn; neighbouring chunks share nodes (their solution gathers and
-  // the partial slots of a shared node, which sit side by side), so XCD x walks the x-th eighth of
-  // the items in order and finds those lines in its own L2.  Items [i0, i1) of XCD x are split over
+  // Workgroups go to the 8 XCDs in turn; neighbouring chunks share nodes (their solution gathers, and
+  // the partial slots of their shared nodes, which the plan numbers in node order, fall on the same
+  // lines), so XCD x walks the x-th eighth of the items in order and finds those lines in its own L2.  Items [i0, i1) of XCD x are split over
   // its nx workgroups with stride nx (every item is taken exactly once).
   // (Grids of fewer than 8 workgroups walk all items with the plain stride.)
   const bool xmap = gridDim.x >= 8;

@@ -409,18 +409,22 @@ static int plan_chunks_host(const Ctx *c, int epc, HostChunkPlan &h) {
       sh_nodes.push_back((int)g);
     }
   }
-  // partial slots, node-major: the slots of shared node k are red_off[k] .. red_off[k+1] - 1, one per
-  // chunk containing it, in chunk order (so the reduction reads contiguous memory)
+  // partial slots: shared node k (in chunks c_0 < c_1 < ...) owns slot k for c_0, n_shared + k for c_1
+  // and 2 n_shared + red_off[k] - 2 k + o - 2 for c_o, o >= 2 (red_off: prefix sums of the chunk
+  // counts), so the reduction finds every shared node's first two partials without a table
   h.node_slot.assign(nodes.size(), -1);
   h.red_off.assign(sh_nodes.size() + 1, 0);
   std::vector<int> &node_slot = h.node_slot, &red_off = h.red_off;
   for (int g : sh_nodes) red_off[sh_index[g] + 1] = touch[g];
   for (size_t k = 0; k < sh_nodes.size(); ++k) red_off[k + 1] += red_off[k];
   {
-    std::vector<int> fillpos(red_off.begin(), red_off.end() - 1);
-    for (size_t k = 0; k < nodes.size(); ++k) {
+    const int nsh = (int)sh_nodes.size();
+    std::vector<int> occ(sh_nodes.size(), 0);
+    for (size_t k = 0; k < nodes.size(); ++k) {  // nodes are listed chunk by chunk, in chunk order
       const int g = nodes[k];
-      if (touch[g] > 1) node_slot[k] = fillpos[sh_index[g]]++;
+      if (touch[g] < 2) continue;
+      const int q = sh_index[g], o = occ[q]++;
+      node_slot[k] = o == 0 ? q : o == 1 ? nsh + q : 2 * nsh + red_off[q] - 2 * q + o - 2;
     }
   }
   h.seg_chunk_off.assign(c->nseg + 1, 0);
@@ -559,16 +563,21 @@ static std::string check_chunk_plan(const Ctx &c, const HostChunkPlan &h) {
     }
   }
   if (iz != h.zero.size() || ish != h.sh_nodes.size()) return "zero_nodes / sh_nodes list extra nodes";
-  std::vector<int> sh_index(Nn, -1), next_slot(h.red_off.begin(), h.red_off.end() - 1);
+  const long long nsh = (long long)h.sh_nodes.size();
+  std::vector<int> sh_index(Nn, -1), occ(h.sh_nodes.size(), 0);
+  std::vector<char> used(h.red_off.back(), 0);
   for (size_t k = 0; k < h.sh_nodes.size(); ++k) sh_index[h.sh_nodes[k]] = (int)k;
   for (int k = 0; k < K; ++k)
     for (int i = h.node_off[k]; i < h.node_off[k + 1]; ++i) {
       const int g = h.nodes[i], sl = h.node_slot[i];
       if (touch[g] == 1) {
         if (sl != -1) return at("exclusive node owns a partial slot", i);
-      } else if (sl != next_slot[sh_index[g]]++) {
-        return at("partial slots of a shared node not contiguous in chunk order", i);
+        continue;
       }
+      const long long q = sh_index[g], o = occ[q]++;
+      const long long want = o == 0 ? q : o == 1 ? nsh + q : 2 * nsh + h.red_off[q] - 2 * q + o - 2;
+      if (sl != want) return at("partial slot of a shared node not at its chunk-order place", i);
+      if (sl < 0 || sl >= (long long)used.size() || used[sl]++) return at("partial slot out of range or reused", i);
     }
   return "";
 }

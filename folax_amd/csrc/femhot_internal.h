@@ -59,7 +59,7 @@ struct ChunkPlan {
   long long n_slots = 0;
   int *node_slot = nullptr;       // per chunk node: partial slot, -1 when the node is exclusive
   int *sh_nodes = nullptr;        // (n_shared) shared node ids, ascending
-  int *red_off = nullptr;         // (n_shared+1) slots of shared node k: red_off[k] .. red_off[k+1]-1
+  int *red_off = nullptr;         // (n_shared+1) prefix sums of the shared nodes' chunk counts
   int *seg_chunk_off = nullptr;   // (nseg+1) chunks of each segment (chunks are segment-sorted)
   // node-centric gather inside a chunk: for each chunk node, the (element position * n + corner)
   // entries touching it, sorted by element position; per chunk the list has n * elements entries
